@@ -1,0 +1,107 @@
+// Phase fold and stable phase sort of one light curve for one trial period, executed by
+// one workgroup.
+//
+// Fold: phase = x - trunc(x), x = t / P in IEEE float64 (the fold of
+// src/gputls/GPUFun.cu:28-39); the quotient is correctly rounded, the subtraction exact,
+// so the keys equal the reference's bit for bit.
+//
+// Sort: the order of (phase, index) pairs -- the permutation a stable argsort of the
+// phases gives (src/gputls/core.py:775-777). One most-significant-digit radix pass
+// distributes the samples over B phase buckets (bucket = floor(phase * B), monotone in
+// the phase): an LDS histogram, an LDS exclusive scan, and a scatter whose slot inside a
+// bucket comes from an LDS atomic. The exact order inside each bucket is then fixed by an
+// insertion sort on the full (phase, index) key, so the scatter order does not matter.
+// With B ~ N/4 a bucket holds a handful of samples, so a period costs one read of t, one
+// scatter and a few compares per sample instead of the 6-8 passes of a 64-bit LSD sort.
+// Phases that crowd into one bucket (periods close to a multiple of the cadence) only
+// make that bucket's insertion sort longer; the result is exact in every case.
+#pragma once
+#include "wg_primitives.h"
+
+namespace gtls {
+
+constexpr int kMaxBuckets = 8192;
+
+__host__ __device__ inline int buckets_for(int64_t n) {
+    int b = 256;
+    while (b < kMaxBuckets && (int64_t)b * 4 < n) b <<= 1;
+    return b;
+}
+
+__device__ __forceinline__ double fold_phase(double t, double P) {
+    const double x = t / P;
+    return x - trunc(x);
+}
+
+// Bucket of a phase in [0, 1), or of (-1, 1) when `neg` (times before zero). The map is
+// monotone non-decreasing in the phase and clamped, so NaN or a wrong `neg` flag cannot
+// index outside the histogram.
+__device__ __forceinline__ int phase_bucket(double ph, int B, bool neg) {
+    double x = neg ? (ph + 1.0) * (0.5 * (double)B) : ph * (double)B;
+    x = fmin(fmax(x, 0.0), (double)(B - 1));
+    return (int)x;
+}
+
+__device__ __forceinline__ bool key_greater(double ka, int32_t ia, double kb, int32_t ib) {
+    return ka > kb || (ka == kb && ia > ib);
+}
+
+// Sorts the n samples of t by (phase, index) for period P. Writes the permutation to
+// perm[0..n) and the sorted phases to key_out[0..n); key_in is n doubles of scratch.
+// hist is LDS of kMaxBuckets ints, red is LDS of blockDim/64 ints. All threads call it.
+__device__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, bool neg,
+                             double* __restrict__ key_in, double* __restrict__ key_out,
+                             int32_t* __restrict__ perm, int* hist, int* red) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int B = buckets_for(n);
+    for (int b = tid; b < B; b += nt) hist[b] = 0;
+    __syncthreads();
+    for (int64_t j = tid; j < n; j += nt) {
+        const double ph = fold_phase(t[j], P);
+        key_in[j] = ph;
+        atomicAdd(&hist[phase_bucket(ph, B, neg)], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the histogram: thread-contiguous chunks, then a workgroup scan
+    {
+        const int per = B / nt;  // B is a multiple of 256 >= blockDim
+        const int base = tid * per;
+        int sum = 0;
+        for (int e = 0; e < per; ++e) sum += hist[base + e];
+        int total;
+        int run = block_exclusive_scan(sum, red, total);
+        for (int e = 0; e < per; ++e) {
+            const int v = hist[base + e];
+            hist[base + e] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    for (int64_t j = tid; j < n; j += nt) {
+        const double ph = key_in[j];
+        const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1);
+        perm[pos] = (int32_t)j;
+        key_out[pos] = ph;
+    }
+    __syncthreads();
+    // hist[b] now holds the end of bucket b; order each bucket by (phase, index)
+    for (int b = tid; b < B; b += nt) {
+        const int lo = b ? hist[b - 1] : 0;
+        const int hi = hist[b];
+        for (int i = lo + 1; i < hi; ++i) {
+            const double k = key_out[i];
+            const int32_t id = perm[i];
+            int j = i - 1;
+            while (j >= lo && key_greater(key_out[j], perm[j], k, id)) {
+                key_out[j + 1] = key_out[j];
+                perm[j + 1] = perm[j];
+                --j;
+            }
+            key_out[j + 1] = k;
+            perm[j + 1] = id;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace gtls

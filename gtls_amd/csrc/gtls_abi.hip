@@ -1,0 +1,347 @@
+// C ABI of the TLS search (declared in include/gtls_amd.h): argument checks, workspace
+// layout and kernel launches. Everything is enqueued on the context's stream; nothing here
+// allocates device memory or waits for the device.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "gtls_amd.h"
+#include "search.h"
+
+struct gtls_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    int slots_fp64 = 0;
+    int slots_fp32 = 0;
+    int slots_sort = 0;
+    std::string err;
+};
+
+namespace {
+
+using namespace gtls;
+
+constexpr int64_t kAlign = 256;
+
+int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+int fail(gtls_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(gtls_ctx* ctx, hipError_t e, const char* where) {
+    if (ctx) ctx->err = std::string(where) + ": " + hipGetErrorString(e);
+    return -(int)e;
+}
+
+struct SearchLayout {
+    int64_t uw0 = 0, tmpl2 = 0, flat = 0, queue = 0, slots = 0;
+    int64_t slot_bytes = 0, o_key_in = 0, o_key_out = 0, o_perm = 0, o_vw = 0, o_D = 0;
+    int64_t n_slots = 0, total = 0;
+};
+
+SearchLayout plan_search(const gtls_ctx* ctx, int mode, int64_t total_points,
+                         int64_t total_tmpl, int32_t n_curves, int64_t max_points,
+                         int64_t max_width) {
+    const int64_t real = (mode == GTLS_MODE_FP64) ? 8 : 4;
+    SearchLayout L;
+    int64_t off = 0;
+    L.uw0 = off;
+    off = align_up(off + 2 * real * total_points);
+    L.tmpl2 = off;
+    off = align_up(off + 2 * real * (total_tmpl > 0 ? total_tmpl : 1));
+    L.flat = off;
+    off = align_up(off + 8 * (int64_t)n_curves);
+    L.queue = off;
+    off = align_up(off + 8);
+    const int64_t np_ = max_points + (max_width > 0 ? max_width : 1);
+    int64_t s = 0;
+    L.o_key_in = s;
+    s = align_up(s + 8 * max_points);
+    L.o_key_out = s;
+    s = align_up(s + 8 * max_points);
+    L.o_perm = s;
+    s = align_up(s + 4 * max_points);
+    L.o_vw = s;
+    s = align_up(s + 2 * real * np_);
+    L.o_D = s;
+    s = align_up(s + 8 * np_);
+    L.slot_bytes = s;
+    L.n_slots = (mode == GTLS_MODE_FP64) ? ctx->slots_fp64 : ctx->slots_fp32;
+    L.slots = off;
+    off += L.slot_bytes * L.n_slots;
+    L.total = align_up(off);
+    return L;
+}
+
+bool batch_ok(const gtls_batch* b, std::string& why) {
+    if (!b) { why = "batch is NULL"; return false; }
+    if (b->n_curves <= 0) { why = "n_curves must be positive"; return false; }
+    if (!b->pt_off || !b->t || !b->y || !b->dy || !b->per_off || !b->periods || !b->k_lo ||
+        !b->k_hi || !b->wid_off || !b->widths || !b->overshoot || !b->tmpl_off || !b->tmpl) {
+        why = "a batch array pointer is NULL";
+        return false;
+    }
+    if (b->total_points <= 0 || b->total_periods < 0 || b->total_widths <= 0 ||
+        b->total_tmpl < 0 || b->max_points <= 0 || b->max_width < 0) {
+        why = "batch sizes out of range";
+        return false;
+    }
+    if (b->max_points > (1LL << 30) || b->max_width > (1LL << 30)) {
+        why = "a light curve or width exceeds 2^30 samples";
+        return false;
+    }
+    if (b->skip_point <= 0) { why = "skip_point must be positive"; return false; }
+    return true;
+}
+
+template <class Real>
+int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayout& L,
+                  double* chi2, int32_t* best_k, int32_t* best_s) {
+    using R2 = typename Pair<Real>::type;
+    hipStream_t st = ctx->stream;
+    R2* uw0 = (R2*)(ws + L.uw0);
+    R2* tmpl2 = (R2*)(ws + L.tmpl2);
+    double* flat = (double*)(ws + L.flat);
+    unsigned long long* queue = (unsigned long long*)(ws + L.queue);
+
+    const int tpb = 256;
+    hipLaunchKernelGGL(prep_points_kernel<Real>, dim3((unsigned)((b->total_points + tpb - 1) / tpb)),
+                       dim3(tpb), 0, st, b->y, b->dy, b->total_points, uw0);
+    if (b->total_tmpl > 0)
+        hipLaunchKernelGGL(prep_template_kernel<Real>,
+                           dim3((unsigned)((b->total_tmpl + tpb - 1) / tpb)), dim3(tpb), 0, st,
+                           b->tmpl, b->total_tmpl, tmpl2);
+    hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
+                       b->y, b->dy, flat);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, st, queue);
+
+    SearchParams<Real> p;
+    p.n_curves = b->n_curves;
+    p.pt_off = b->pt_off;
+    p.t = b->t;
+    p.per_off = b->per_off;
+    p.periods = b->periods;
+    p.k_lo = b->k_lo;
+    p.k_hi = b->k_hi;
+    p.wid_off = b->wid_off;
+    p.widths = b->widths;
+    p.overshoot = b->overshoot;
+    p.tmpl_off = b->tmpl_off;
+    p.uw0 = uw0;
+    p.tmpl2 = tmpl2;
+    p.chi2_flat = flat;
+    p.skip_point = b->skip_point;
+    p.depth_min = b->depth_min;
+    p.negative_time = b->negative_time;
+    p.total_periods = b->total_periods;
+    p.max_points = b->max_points;
+    p.max_width = b->max_width > 0 ? b->max_width : 1;
+    p.queue = queue;
+    p.slots = ws + L.slots;
+    p.slot_bytes = L.slot_bytes;
+    p.o_key_in = L.o_key_in;
+    p.o_key_out = L.o_key_out;
+    p.o_perm = L.o_perm;
+    p.o_vw = L.o_vw;
+    p.o_D = L.o_D;
+    p.chi2 = chi2;
+    p.best_k = best_k;
+    p.best_s = best_s;
+    if (b->total_periods > 0) {
+        const int64_t grid = std::min<int64_t>(L.n_slots, b->total_periods);
+        hipLaunchKernelGGL(tls_search_kernel<Real>, dim3((unsigned)grid), dim3(kBlock), 0, st, p);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "search launch");
+    return GTLS_OK;
+}
+
+int occupancy_slots(gtls_ctx* ctx, const void* kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return per_cu * ctx->n_cu;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gtls_create(int device, void* stream, gtls_ctx** out) {
+    if (!out) return GTLS_E_SHAPE;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return -(int)e;
+    if (device < 0 || device >= count) return GTLS_E_DEVICE;
+    DeviceGuard guard(device);
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return -(int)e;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GTLS_E_DEVICE;
+    gtls_ctx* ctx = new gtls_ctx();
+    ctx->device = device;
+    ctx->stream = (hipStream_t)stream;
+    ctx->n_cu = prop.multiProcessorCount;
+    ctx->slots_fp64 = occupancy_slots(ctx, (const void*)tls_search_kernel<double>);
+    ctx->slots_fp32 = occupancy_slots(ctx, (const void*)tls_search_kernel<float>);
+    ctx->slots_sort = occupancy_slots(ctx, (const void*)fold_sort_kernel);
+    *out = ctx;
+    return GTLS_OK;
+}
+
+void gtls_destroy(gtls_ctx* ctx) { delete ctx; }
+
+int gtls_set_stream(gtls_ctx* ctx, void* stream) {
+    if (!ctx) return GTLS_E_SHAPE;
+    ctx->stream = (hipStream_t)stream;
+    return GTLS_OK;
+}
+
+const char* gtls_last_error(const gtls_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int32_t gtls_search_slots(const gtls_ctx* ctx) { return ctx ? ctx->slots_fp32 : 0; }
+
+int64_t gtls_search_workspace_bytes(gtls_ctx* ctx, int32_t mode, const gtls_batch* b) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C) return fail(ctx, GTLS_E_MODE, "unknown mode");
+    std::string why;
+    if (!batch_ok(b, why)) return fail(ctx, GTLS_E_SHAPE, why);
+    return plan_search(ctx, mode, b->total_points, b->total_tmpl, b->n_curves, b->max_points,
+                       b->max_width).total;
+}
+
+int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* b, void* workspace,
+                        int64_t workspace_bytes, double* chi2, int32_t* best_k,
+                        int32_t* best_s) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C) return fail(ctx, GTLS_E_MODE, "unknown mode");
+    std::string why;
+    if (!batch_ok(b, why)) return fail(ctx, GTLS_E_SHAPE, why);
+    if (!workspace || !chi2 || !best_k || !best_s) return fail(ctx, GTLS_E_SHAPE, "NULL output or workspace");
+    DeviceGuard guard(ctx->device);
+    const SearchLayout L = plan_search(ctx, mode, b->total_points, b->total_tmpl, b->n_curves,
+                                       b->max_points, b->max_width);
+    if (workspace_bytes < L.total)
+        return fail(ctx, GTLS_E_WORKSPACE, "workspace too small: need " + std::to_string(L.total) +
+                                               " bytes, got " + std::to_string(workspace_bytes));
+    if (mode == GTLS_MODE_FP64)
+        return launch_search<double>(ctx, b, (char*)workspace, L, chi2, best_k, best_s);
+    return launch_search<float>(ctx, b, (char*)workspace, L, chi2, best_k, best_s);
+}
+
+// Single curve: its CSR offsets are written into the head of the workspace by a tiny
+// device kernel, so the call stays free of host-to-device copies.
+namespace {
+__global__ void single_offsets_kernel(int64_t* o, int64_t n, int64_t np, int64_t nw) {
+    o[0] = 0; o[1] = n;
+    o[2] = 0; o[3] = np;
+    o[4] = 0; o[5] = nw;
+}
+constexpr int64_t kSingleHead = 256;
+}  // namespace
+
+int64_t gtls_search_workspace_bytes_1(gtls_ctx* ctx, int32_t mode, int64_t n,
+                                      int64_t n_periods, int32_t n_widths,
+                                      int64_t total_tmpl, int32_t max_width) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C) return fail(ctx, GTLS_E_MODE, "unknown mode");
+    if (n <= 0 || n_periods < 0 || n_widths <= 0 || total_tmpl < 0 || max_width < 0 ||
+        n > (1LL << 30))
+        return fail(ctx, GTLS_E_SHAPE, "sizes out of range");
+    return kSingleHead + plan_search(ctx, mode, n, total_tmpl, 1, n, max_width).total;
+}
+
+int gtls_search(gtls_ctx* ctx, int32_t mode, const double* t, const double* y,
+                const double* dy, int64_t n, const double* periods, int64_t n_periods,
+                const int32_t* k_lo, const int32_t* k_hi, const int32_t* widths,
+                int32_t n_widths, int32_t max_width, const double* overshoot,
+                const int64_t* tmpl_off, const double* tmpl, int64_t total_tmpl,
+                int64_t skip_point, double depth_min, int32_t negative_time, void* workspace,
+                int64_t workspace_bytes, double* chi2, int32_t* best_k, int32_t* best_s) {
+    if (!ctx) return GTLS_E_SHAPE;
+    int64_t need = gtls_search_workspace_bytes_1(ctx, mode, n, n_periods, n_widths, total_tmpl,
+                                                 max_width);
+    if (need < 0) return (int)need;
+    if (!workspace) return fail(ctx, GTLS_E_SHAPE, "NULL workspace");
+    if (workspace_bytes < need)
+        return fail(ctx, GTLS_E_WORKSPACE, "workspace too small: need " + std::to_string(need) +
+                                               " bytes, got " + std::to_string(workspace_bytes));
+    DeviceGuard guard(ctx->device);
+    int64_t* head = (int64_t*)workspace;
+    hipLaunchKernelGGL(single_offsets_kernel, dim3(1), dim3(1), 0, ctx->stream, head, n,
+                       n_periods, (int64_t)n_widths);
+    gtls_batch b;
+    std::memset(&b, 0, sizeof(b));
+    b.n_curves = 1;
+    b.pt_off = head;
+    b.t = t;
+    b.y = y;
+    b.dy = dy;
+    b.per_off = head + 2;
+    b.periods = periods;
+    b.k_lo = k_lo;
+    b.k_hi = k_hi;
+    b.wid_off = head + 4;
+    b.widths = widths;
+    b.overshoot = overshoot;
+    b.tmpl_off = tmpl_off;
+    b.tmpl = tmpl;
+    b.skip_point = skip_point;
+    b.depth_min = depth_min;
+    b.negative_time = negative_time;
+    b.total_points = n;
+    b.total_periods = n_periods;
+    b.total_widths = n_widths;
+    b.total_tmpl = total_tmpl;
+    b.max_points = (int32_t)n;
+    b.max_width = max_width;
+    return gtls_search_batched(ctx, mode, &b, (char*)workspace + kSingleHead,
+                               workspace_bytes - kSingleHead, chi2, best_k, best_s);
+}
+
+int64_t gtls_fold_sort_workspace_bytes(gtls_ctx* ctx, int64_t n) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (n <= 0 || n > (1LL << 30)) return fail(ctx, GTLS_E_SHAPE, "n out of range");
+    return align_up(16 * n) * ctx->slots_sort;
+}
+
+int gtls_fold_sort(gtls_ctx* ctx, const double* t, int64_t n, const double* periods,
+                   int64_t n_periods, int32_t negative_time, void* workspace,
+                   int64_t workspace_bytes, int32_t* perm) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (!t || !periods || !perm || !workspace) return fail(ctx, GTLS_E_SHAPE, "NULL pointer");
+    int64_t need = gtls_fold_sort_workspace_bytes(ctx, n);
+    if (need < 0) return (int)need;
+    if (n_periods < 0) return fail(ctx, GTLS_E_SHAPE, "n_periods < 0");
+    if (workspace_bytes < need) return fail(ctx, GTLS_E_WORKSPACE, "workspace too small");
+    if (n_periods == 0) return GTLS_OK;
+    DeviceGuard guard(ctx->device);
+    const int64_t grid = std::min<int64_t>(ctx->slots_sort, n_periods);
+    hipLaunchKernelGGL(fold_sort_kernel, dim3((unsigned)grid), dim3(kBlock), 0, ctx->stream, t,
+                       n, periods, n_periods, negative_time, (char*)workspace, align_up(16 * n),
+                       perm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "fold_sort launch");
+    return GTLS_OK;
+}
+
+}  // extern "C"

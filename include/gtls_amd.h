@@ -1,0 +1,134 @@
+/*
+ * gtls_amd -- C ABI of the MI355X-native TLS chi2-vs-period search.
+ *
+ * Every array argument is a DEVICE pointer (memory the caller owns, e.g. a torch tensor
+ * on the same device); sizes and scalars are passed by value. Nothing in this ABI
+ * allocates device memory or synchronises: all work is enqueued on the context's
+ * stream, into the caller's workspace, so a call can be captured into a HIP graph.
+ *
+ * Return value: 0 on success, GTLS_E_* (negative) on a library error, or
+ * -(hipError_t) when a HIP call fails. gtls_last_error() gives a message.
+ *
+ * Reference interfaces these entry points replace (paths into tomarriott/GTLS):
+ *   gtls_search / gtls_search_batched
+ *       the per-batch kernel chain of src/gputls/core.py:691-889 (search_multi_periods):
+ *       foldFast (src/gputls/GPUFun.cu:28-39), CuPy argsort (core.py:775-777),
+ *       patchData (GPUFun.cu:108-127), calcInverseSquaredPatchedDy (GPUFun.cu:132-140),
+ *       calcEdgeEffectCorrections (GPUFun.cu:145-166), cp.cumsum (core.py:799-822),
+ *       calculate_base_error / calcAllFullSum_v2 / calculate_final_ootr_v3
+ *       (GPUFun.cu:205-357), calcAllLowestResidualsGPUB_SignalTiled_v2
+ *       (GPUFun.cu:380-436) and cp.argmin (core.py:866-875); the no-skip variants of the
+ *       re-search (core.py:1001-1247, GPUFun.cu:817-871) are the same call with
+ *       skip_point = GTLS_SKIP_NONE. The batched form runs many light curves, each with
+ *       its own period grid and width table, in one persistent launch.
+ *   gtls_fold_sort
+ *       foldFast + argsort (GPUFun.cu:28-39, core.py:775-777): the stable phase-order
+ *       permutation of every trial period, exposed for bit-exact checks.
+ *   gtls_spectra_batched
+ *       spectra() of src/gputls/stats.py:95-122 (SR, power_raw, detrended power, SDE)
+ *       with the running median of src/gputls/helpers.py:91-106, per light curve.
+ */
+#ifndef GTLS_AMD_H
+#define GTLS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GTLS_OK 0
+#define GTLS_E_SHAPE (-1001)     /* inconsistent sizes or a null pointer */
+#define GTLS_E_WORKSPACE (-1002) /* workspace smaller than gtls_*_workspace_bytes() */
+#define GTLS_E_MODE (-1003)      /* unknown arithmetic mode */
+#define GTLS_E_DEVICE (-1004)    /* device not usable / not gfx950 */
+
+#define GTLS_MODE_FP64 0  /* float64 throughout */
+#define GTLS_MODE_FP32C 1 /* float64 fold, prefix sums and final chi2; float32 windows */
+
+#define GTLS_SKIP_NONE 0x7f800000LL /* skip_point value for T0_fit_margin == 0 */
+
+typedef struct gtls_ctx gtls_ctx;
+
+/*
+ * A batch of light curves in CSR layout. Curve c owns points [pt_off[c], pt_off[c+1]),
+ * periods [per_off[c], per_off[c+1]) and widths [wid_off[c], wid_off[c+1]).
+ * For period g of curve c the searched widths are the curve-local indices
+ * k_lo[g]..k_hi[g] (inclusive; empty if k_lo > k_hi) of the curve's ascending width
+ * table. Width k (global index wid_off[c] + k) has template row
+ * tmpl[tmpl_off[.] .. tmpl_off[.] + widths[.]) holding S = 1 - signal and overshoot
+ * factor overshoot[.]. Pointers are device pointers; the size fields are host values.
+ */
+typedef struct gtls_batch {
+    int32_t n_curves;
+    const int64_t* pt_off;   /* [n_curves + 1] */
+    const double* t;         /* [total_points] time, days */
+    const double* y;         /* [total_points] normalised flux */
+    const double* dy;        /* [total_points] flux error */
+    const int64_t* per_off;  /* [n_curves + 1] */
+    const double* periods;   /* [total_periods] days */
+    const int32_t* k_lo;     /* [total_periods] */
+    const int32_t* k_hi;     /* [total_periods] */
+    const int64_t* wid_off;  /* [n_curves + 1] */
+    const int32_t* widths;   /* [total_widths] samples, ascending within a curve */
+    const double* overshoot; /* [total_widths] */
+    const int64_t* tmpl_off; /* [total_widths] */
+    const double* tmpl;      /* [total_tmpl] */
+    int64_t skip_point;      /* int(1 / T0_fit_margin), or GTLS_SKIP_NONE */
+    double depth_min;        /* transit_depth_min */
+    int32_t negative_time;   /* nonzero if any t < 0 (changes the bucket map only) */
+    /* host-side sizes (must match the device arrays) */
+    int64_t total_points;
+    int64_t total_periods;
+    int64_t total_widths;
+    int64_t total_tmpl;
+    int32_t max_points;      /* max over curves of the point count */
+    int32_t max_width;       /* max over all widths */
+} gtls_batch;
+
+/* Context bound to one device and one stream (hipStream_t, may be NULL). */
+int gtls_create(int device, void* stream, gtls_ctx** out);
+void gtls_destroy(gtls_ctx* ctx);
+int gtls_set_stream(gtls_ctx* ctx, void* stream);
+const char* gtls_last_error(const gtls_ctx* ctx);
+/* Persistent workgroups the search launches on this device (one scratch slot each). */
+int32_t gtls_search_slots(const gtls_ctx* ctx);
+
+/* Workspace bytes gtls_search_batched needs for this batch and mode (>= 0), or < 0. */
+int64_t gtls_search_workspace_bytes(gtls_ctx* ctx, int32_t mode, const gtls_batch* batch);
+
+/*
+ * chi2-vs-period search of every period of every curve of the batch.
+ * Outputs per period g (device arrays of length total_periods): chi2[g] = minimum of the
+ * TLS statistic over (width, T0 position) (NaN if the period has no widths),
+ * best_k[g] = curve-local width index of that minimum, best_s[g] = its position in the
+ * phase-sorted sequence; ties resolve to the first in (width, position) order.
+ */
+int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* batch,
+                        void* workspace, int64_t workspace_bytes,
+                        double* chi2, int32_t* best_k, int32_t* best_s);
+
+/* Single light curve: the same search with the arrays of one curve (device pointers). */
+int64_t gtls_search_workspace_bytes_1(gtls_ctx* ctx, int32_t mode, int64_t n,
+                                      int64_t n_periods, int32_t n_widths,
+                                      int64_t total_tmpl, int32_t max_width);
+int gtls_search(gtls_ctx* ctx, int32_t mode,
+                const double* t, const double* y, const double* dy, int64_t n,
+                const double* periods, int64_t n_periods,
+                const int32_t* k_lo, const int32_t* k_hi,
+                const int32_t* widths, int32_t n_widths, int32_t max_width,
+                const double* overshoot, const int64_t* tmpl_off, const double* tmpl,
+                int64_t total_tmpl, int64_t skip_point, double depth_min,
+                int32_t negative_time, void* workspace, int64_t workspace_bytes,
+                double* chi2, int32_t* best_k, int32_t* best_s);
+
+/* Phase-order permutation of t for each period: perm[p * n + j] (stable in index). */
+int64_t gtls_fold_sort_workspace_bytes(gtls_ctx* ctx, int64_t n);
+int gtls_fold_sort(gtls_ctx* ctx, const double* t, int64_t n, const double* periods,
+                   int64_t n_periods, int32_t negative_time,
+                   void* workspace, int64_t workspace_bytes, int32_t* perm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GTLS_AMD_H */
