@@ -1,0 +1,155 @@
+"""Batched search: many light curves, each with its own period grid and template table, in
+one persistent launch (gtls_search_batched).
+
+Curves are packed in CSR layout (points, periods, widths, template samples), largest
+first so that the device work queue hands out the longest items early, and the
+per-period outputs are mapped back to the caller's curve order.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+from . import _ffi
+from . import constants as C
+from .search import MODES, _ctx, _device, _ptr, make_plan
+from .validate import validate_args, validate_inputs
+
+
+def _plan_one(item):
+    t, y, dy, kw = item
+    t, y, dy = validate_inputs(t, y, dy)
+    return make_plan(t, y, dy, validate_args(kw))
+
+
+def build_plans(curves, kwargs=None, workers=None):
+    """CurvePlans of (t, y, dy) tuples, built in a process pool."""
+    kwargs = kwargs or {}
+    items = [(c[0], c[1], c[2], kwargs) for c in curves]
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(items) < 8:
+        return [_plan_one(it) for it in items]
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(_plan_one, items, chunksize=max(1, len(items) // (4 * workers))))
+
+
+class BatchPlan:
+    """Host CSR arrays of a list of CurvePlans."""
+
+    def __init__(self, plans):
+        self.plans = plans
+        n = len(plans)
+        work = np.array([p.n * len(p.periods) * max(1, int(np.mean(p.k_hi - p.k_lo + 1)))
+                         for p in plans], dtype=np.float64)
+        self.order = np.argsort(-work, kind="stable")      # packed position -> curve
+        ps = [plans[i] for i in self.order]
+        self.n_curves = n
+        self.pt_off = np.zeros(n + 1, np.int64)
+        self.per_off = np.zeros(n + 1, np.int64)
+        self.wid_off = np.zeros(n + 1, np.int64)
+        self.pt_off[1:] = np.cumsum([p.n for p in ps])
+        self.per_off[1:] = np.cumsum([len(p.periods) for p in ps])
+        self.wid_off[1:] = np.cumsum([p.table.n_widths for p in ps])
+        self.t = np.concatenate([p.t for p in ps])
+        self.y = np.concatenate([p.y for p in ps])
+        self.dy = np.concatenate([p.dy for p in ps])
+        self.periods = np.concatenate([p.periods for p in ps])
+        self.k_lo = np.concatenate([p.k_lo for p in ps]).astype(np.int32)
+        self.k_hi = np.concatenate([p.k_hi for p in ps]).astype(np.int32)
+        self.widths = np.concatenate([p.table.widths for p in ps]).astype(np.int32)
+        self.overshoot = np.concatenate([p.table.overshoot for p in ps])
+        tbase = np.cumsum([0] + [len(p.table.S) for p in ps[:-1]])
+        self.tmpl_off = np.concatenate([p.table.offsets[:-1] + b for p, b in zip(ps, tbase)])
+        self.tmpl = np.concatenate([p.table.S for p in ps])
+        self.max_points = int(max(p.n for p in ps))
+        self.max_width = int(self.widths.max())
+        self.negative_time = int(np.min(self.t) < 0)
+
+    @property
+    def total_periods(self):
+        return int(self.per_off[-1])
+
+    def unpack(self, arr):
+        """Per-curve arrays (caller order) of a packed per-period array."""
+        out = [None] * self.n_curves
+        for pos, ci in enumerate(self.order):
+            out[ci] = arr[self.per_off[pos]:self.per_off[pos + 1]]
+        return out
+
+    def work_points(self):
+        """Sum over (curve, period) of N' = N + widest searched width (for roofline)."""
+        s = 0
+        for p in self.plans:
+            wmax = p.table.widths[np.clip(p.k_hi, 0, None)]
+            ok = p.k_lo <= p.k_hi
+            s += int(np.sum((p.n + wmax)[ok]))
+        return s
+
+
+class DeviceBatch:
+    """Device-resident BatchPlan; search() enqueues one persistent search launch."""
+
+    def __init__(self, bp: BatchPlan, device=0):
+        import torch
+        self.bp = bp
+        dev = _device(device)
+        self.dev = dev
+
+        def f64(a):
+            return torch.as_tensor(np.ascontiguousarray(a, np.float64), device=dev)
+
+        def i64(a):
+            return torch.as_tensor(np.ascontiguousarray(a, np.int64), device=dev)
+
+        def i32(a):
+            return torch.as_tensor(np.ascontiguousarray(a, np.int32), device=dev)
+
+        self.arrays = dict(
+            pt_off=i64(bp.pt_off), t=f64(bp.t), y=f64(bp.y), dy=f64(bp.dy),
+            per_off=i64(bp.per_off), periods=f64(bp.periods), k_lo=i32(bp.k_lo),
+            k_hi=i32(bp.k_hi), wid_off=i64(bp.wid_off), widths=i32(bp.widths),
+            overshoot=f64(bp.overshoot), tmpl_off=i64(bp.tmpl_off), tmpl=f64(bp.tmpl))
+        npd = bp.total_periods
+        self.chi2 = torch.empty(npd, dtype=torch.float64, device=dev)
+        self.best_k = torch.empty(npd, dtype=torch.int32, device=dev)
+        self.best_s = torch.empty(npd, dtype=torch.int32, device=dev)
+        self._ws = None
+
+    def batch_struct(self, skip_point, depth_min):
+        a, bp = self.arrays, self.bp
+        b = _ffi.GtlsBatch()
+        b.n_curves = bp.n_curves
+        for name in ("pt_off", "t", "y", "dy", "per_off", "periods", "k_lo", "k_hi",
+                     "wid_off", "widths", "overshoot", "tmpl_off", "tmpl"):
+            setattr(b, name, a[name].data_ptr())
+        b.skip_point = int(skip_point)
+        b.depth_min = float(depth_min)
+        b.negative_time = bp.negative_time
+        b.total_points = int(bp.pt_off[-1])
+        b.total_periods = bp.total_periods
+        b.total_widths = int(bp.wid_off[-1])
+        b.total_tmpl = int(len(bp.tmpl))
+        b.max_points = bp.max_points
+        b.max_width = bp.max_width
+        return b
+
+    def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
+        """Enqueue the batched search on torch's current stream (no synchronisation)."""
+        import torch
+        m = MODES[mode]
+        ctx = _ctx(self.dev)
+        b = self.batch_struct(skip_point, depth_min)
+        lib = _ffi.load()
+        need = lib.gtls_search_workspace_bytes(ctx.handle, m, ct.byref(b))
+        if need < 0:
+            _ffi.check(ctx.handle, int(need))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(int(need), dtype=torch.uint8, device=self.dev)
+        code = lib.gtls_search_batched(ctx.handle, m, ct.byref(b), _ptr(self._ws),
+                                       self._ws.numel(), _ptr(self.chi2), _ptr(self.best_k),
+                                       _ptr(self.best_s))
+        _ffi.check(ctx.handle, code)
+        return self.chi2, self.best_k, self.best_s

@@ -9,6 +9,7 @@
 
 #include "gtls_amd.h"
 #include "search.h"
+#include "spectra.h"
 
 struct gtls_ctx {
     int device = 0;
@@ -341,6 +342,48 @@ int gtls_fold_sort(gtls_ctx* ctx, const double* t, int64_t n, const double* peri
                        perm);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "fold_sort launch");
+    return GTLS_OK;
+}
+
+int64_t gtls_spectra_workspace_bytes(gtls_ctx* ctx, int64_t total_periods) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (total_periods < 0) return fail(ctx, GTLS_E_SHAPE, "total_periods < 0");
+    return align_up(8 * (total_periods > 0 ? total_periods : 1));
+}
+
+int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off,
+                         const double* chi2, int64_t total_periods, int32_t kernel,
+                         void* workspace, int64_t workspace_bytes, double* power_raw,
+                         double* power, double* sde_raw, double* sde, int32_t* status) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (n_curves <= 0 || total_periods < 0 || !per_off || !chi2 || !workspace || !power_raw ||
+        !power || !sde_raw || !sde || !status)
+        return fail(ctx, GTLS_E_SHAPE, "bad spectra arguments");
+    if (kernel < 1 || kernel > kMaxMedianKernel || kernel % 2 == 0)
+        return fail(ctx, GTLS_E_SHAPE, "median kernel must be odd and <= 255");
+    if (workspace_bytes < gtls_spectra_workspace_bytes(ctx, total_periods))
+        return fail(ctx, GTLS_E_WORKSPACE, "workspace too small");
+    DeviceGuard guard(ctx->device);
+    SpecParams p;
+    p.n_curves = n_curves;
+    p.per_off = per_off;
+    p.chi2 = chi2;
+    p.kernel = kernel;
+    p.power_raw = power_raw;
+    p.power = power;
+    p.sde_raw = sde_raw;
+    p.sde = sde;
+    p.status = status;
+    double* med = (double*)workspace;
+    hipLaunchKernelGGL(spectra_pass1_kernel, dim3(n_curves), dim3(kSpecBlock), 0, ctx->stream, p);
+    if (total_periods > 0)
+        hipLaunchKernelGGL(running_median_kernel,
+                           dim3((unsigned)((total_periods + kSpecBlock - 1) / kSpecBlock)),
+                           dim3(kSpecBlock), 0, ctx->stream, p, total_periods, med);
+    hipLaunchKernelGGL(spectra_pass2_kernel, dim3(n_curves), dim3(kSpecBlock), 0, ctx->stream, p,
+                       (const double*)med);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "spectra launch");
     return GTLS_OK;
 }
 

@@ -128,6 +128,20 @@ int gtls_fold_sort(gtls_ctx* ctx, const double* t, int64_t n, const double* peri
                    int64_t n_periods, int32_t negative_time,
                    void* workspace, int64_t workspace_bytes, int32_t* perm);
 
+/*
+ * SR / SDE spectra of every curve of a batch from its chi2 vector (CSR by per_off):
+ * power_raw, power (detrended with a running median of odd width `kernel`, applied when a
+ * curve has more than 2 * kernel periods), SDE_raw and SDE per curve. status[c] = 0 when
+ * the device computed curve c, 1 when the curve needs the host path (a NaN chi2, or
+ * max chi2 > 100 * min chi2, the only case in which the outlier mask can be non-empty).
+ * Workspace: gtls_spectra_workspace_bytes(total_periods).
+ */
+int64_t gtls_spectra_workspace_bytes(gtls_ctx* ctx, int64_t total_periods);
+int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off,
+                         const double* chi2, int64_t total_periods, int32_t kernel,
+                         void* workspace, int64_t workspace_bytes, double* power_raw,
+                         double* power, double* sde_raw, double* sde, int32_t* status);
+
 #ifdef __cplusplus
 }
 #endif
