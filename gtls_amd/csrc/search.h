@@ -96,39 +96,37 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
     for (;;) {
         if (tid == 0) s_item = atomicAdd(p.queue, 1ULL);
         __syncthreads();
-        const int64_t g = (int64_t)s_item;
+        const int64_t g = uniform((int64_t)s_item);
         __syncthreads();
         if (g >= p.total_periods) break;
 
+        // Every value below derives from g and is wave-uniform; the item either runs all
+        // three stages or only writes its status, and both paths meet at the barrier
+        // that ends the iteration (no early continue out of a loop with barriers).
         const int c = find_owner(p.per_off, p.n_curves, g);
         const int64_t pt0 = p.pt_off[c];
         const int64_t n = p.pt_off[c + 1] - pt0;
         const int64_t w0 = p.wid_off[c];
         const int32_t klo = p.k_lo[g], khi = p.k_hi[g];
-        if (klo > khi || n <= 0) {
-            if (tid == 0) {
-                p.chi2[g] = __builtin_nan("");
-                p.best_k[g] = -1;
-                p.best_s[g] = -1;
-            }
-            continue;
-        }
-        // inputs that would overrun the slot: report (-2) instead of touching memory
-        const int64_t nw = p.wid_off[c + 1] - w0;
+        // status: 0 run, -1 no widths (NaN), -2 inputs that would overrun the slot
+        int32_t status = (klo > khi || n <= 0) ? -1 : 0;
         int32_t dmax = 1;
-        bool bad = klo < 0 || khi >= nw || n > p.max_points;
-        if (!bad) {
-            for (int32_t k = klo; k <= khi; ++k) dmax = max(dmax, p.widths[w0 + k]);
-            bad = dmax > p.max_width;
+        if (status == 0) {
+            const int64_t nw = p.wid_off[c + 1] - w0;
+            if (klo < 0 || khi >= nw || n > p.max_points) {
+                status = -2;
+            } else {
+                for (int32_t k = klo; k <= khi; ++k) dmax = max(dmax, p.widths[w0 + k]);
+                if (dmax > p.max_width) status = -2;
+            }
         }
-        if (bad) {
+        if (status != 0) {
             if (tid == 0) {
                 p.chi2[g] = __builtin_nan("");
-                p.best_k[g] = -2;
-                p.best_s[g] = -2;
+                p.best_k[g] = status;
+                p.best_s[g] = status;
             }
-            continue;
-        }
+        } else {
         const double P = p.periods[g];
 
         // 1. fold + sort
@@ -223,6 +221,7 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
             p.best_k[g] = found ? best.k : -1;
             p.best_s[g] = found ? best.s : -1;
         }
+        }  // status == 0
         __syncthreads();
     }
 }

@@ -73,9 +73,9 @@ __global__ __launch_bounds__(kSpecBlock) void spectra_pass1_kernel(SpecParams p)
         if (x != x) nans += 1.0;
         else { lo = fmin(lo, x); hi = fmax(hi, x); }
     }
-    lo = block_reduce_min(lo, tmp);
-    hi = block_reduce_max(hi, tmp);
-    nans = block_reduce_sum(nans, tmp);
+    lo = uniform(block_reduce_min(lo, tmp));
+    hi = uniform(block_reduce_max(hi, tmp));
+    nans = uniform(block_reduce_sum(nans, tmp));
     if (nans > 0.0 || !(hi <= 100.0 * lo)) {
         if (tid == 0) p.status[c] = 1;
         return;

@@ -8,6 +8,21 @@ namespace gtls {
 constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Values every lane holds identically (read back from LDS after a barrier) moved into
+// scalar registers, so that branches on them compile to uniform scalar branches. Loop
+// control of the persistent kernels must be uniform: a branch the compiler believes to
+// be divergent can be restructured so that lanes of one wavefront reach different
+// instances of a barrier.
+__device__ __forceinline__ int32_t uniform(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uniform(int64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double uniform(double x) {
+    return __builtin_bit_cast(double, uniform(__builtin_bit_cast(int64_t, x)));
+}
 __device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
 
 // Inclusive prefix sum across the 64 lanes of a wavefront.
