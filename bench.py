@@ -89,9 +89,7 @@ def main():
     from gtls_amd import synthetic
     from gtls_amd import search as S
     from gtls_amd import spectra as SP
-    from gtls_amd import _ffi
     from gtls_amd.grid import skip_factor
-    import ctypes as ct
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
@@ -105,31 +103,13 @@ def main():
     db = B.DeviceBatch(bp, 0)
     skip = skip_factor(0.125)
 
-    # device spectra buffers
-    npd = bp.total_periods
-    per_off_d = db.arrays["per_off"]
-    power_raw = torch.empty(npd, dtype=torch.float64, device=dev)
-    power = torch.empty(npd, dtype=torch.float64, device=dev)
-    sde_raw = torch.empty(bp.n_curves, dtype=torch.float64, device=dev)
-    sde = torch.empty(bp.n_curves, dtype=torch.float64, device=dev)
-    status = torch.empty(bp.n_curves, dtype=torch.int32, device=dev)
-    lib = _ffi.load()
-    ctx = S._ctx(dev)
-    spec_ws = torch.empty(int(lib.gtls_spectra_workspace_bytes(ctx.handle, npd)),
-                          dtype=torch.uint8, device=dev)
-    kernel = SP.median_kernel(3)
-
     def step(ev=None):
         if ev is not None:
             ev[0].record()
         chi2, _, _ = db.search("fp32c", skip)
         if ev is not None:
             ev[1].record()
-        code = lib.gtls_spectra_batched(ctx.handle, bp.n_curves, S._ptr(per_off_d),
-                                        S._ptr(chi2), npd, kernel, S._ptr(spec_ws),
-                                        spec_ws.numel(), S._ptr(power_raw), S._ptr(power),
-                                        S._ptr(sde_raw), S._ptr(sde), S._ptr(status))
-        _ffi.check(ctx.handle, code)
+        db.spectra(chi2, 3, sync=False)
 
     for i in range(args.warmup):
         step()
@@ -156,12 +136,11 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "kernel": "tls_search_kernel<float>", "algorithmic_bytes": algo}
 
-    chi2_b = bp.unpack(db.chi2.cpu().numpy())
     bk_b = bp.unpack(db.best_k.cpu().numpy())
-    pw_b = bp.unpack(power.cpu().numpy())
-    sde_b = sde.cpu().numpy()[np.argsort(bp.order)]
-    st_b = status.cpu().numpy()[np.argsort(bp.order)]
-    extra = {"device_spectra_curves": int(np.sum(st_b == 0))}
+    spec = db.spectra(db.chi2, 3)          # finishes host fallbacks of the last step
+    pw_b, sde_b = spec["power"], spec["SDE"]
+    extra = {"device_spectra_curves": int(np.sum(spec["device"]))}
+    npd = bp.total_periods
 
     total_work = sum(curve_work(p) for p in plans)
 
@@ -182,11 +161,10 @@ def main():
             t_plain += time.perf_counter() - ts
             w_plain += curve_work(p)
             kk = k.cpu().numpy()
-            ib, ip = int(np.argmax(pw_b[ci])), int(np.argmax(np.asarray(pw)))
+            ib, ip = int(np.argmax(np.asarray(pw_b[ci]))), int(np.argmax(np.asarray(pw)))
             same_period += ib == ip
             same_width += (ib == ip) and bk_b[ci][ib] == kk[ip]
-            sdb = sde_b[ci] if st_b[ci] == 0 else np.nan
-            max_dsde = max(max_dsde, abs(sdb - sd) / abs(sd))
+            max_dsde = max(max_dsde, abs(float(sde_b[ci]) - float(sd)) / abs(float(sd)))
         plain_value = len(plans) / (t_plain * total_work / w_plain) * 3600.0
         extra.update({"plain_fp64_curves_per_hour": round(plain_value, 1),
                       "speedup_vs_plain_fp64": round(value / plain_value, 3),

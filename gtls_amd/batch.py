@@ -136,6 +136,51 @@ class DeviceBatch:
         b.max_width = bp.max_width
         return b
 
+    def spectra(self, chi2, oversampling_factor=C.OVERSAMPLING_FACTOR, sync=True):
+        """Device SR/SDE spectra of every curve (gtls_spectra_batched); curves the device
+        hands back (status 1) are recomputed on the host. Returns per-curve lists
+        (caller order) of power, power_raw, SDE and SDE_raw."""
+        import torch
+        from . import spectra as SP
+        bp = self.bp
+        npd = bp.total_periods
+        if not hasattr(self, "_spec"):
+            dev = self.dev
+            lib = _ffi.load()
+            ctx = _ctx(dev)
+            self._spec = dict(
+                pr=torch.empty(npd, dtype=torch.float64, device=dev),
+                pw=torch.empty(npd, dtype=torch.float64, device=dev),
+                sde_raw=torch.empty(bp.n_curves, dtype=torch.float64, device=dev),
+                sde=torch.empty(bp.n_curves, dtype=torch.float64, device=dev),
+                status=torch.empty(bp.n_curves, dtype=torch.int32, device=dev),
+                ws=torch.empty(int(lib.gtls_spectra_workspace_bytes(ctx.handle, npd)),
+                               dtype=torch.uint8, device=dev))
+        sp = self._spec
+        lib = _ffi.load()
+        ctx = _ctx(self.dev)
+        code = lib.gtls_spectra_batched(
+            ctx.handle, bp.n_curves, _ptr(self.arrays["per_off"]), _ptr(chi2), npd,
+            SP.median_kernel(oversampling_factor), _ptr(sp["ws"]), sp["ws"].numel(),
+            _ptr(sp["pr"]), _ptr(sp["pw"]), _ptr(sp["sde_raw"]), _ptr(sp["sde"]),
+            _ptr(sp["status"]))
+        _ffi.check(ctx.handle, code)
+        if not sync:
+            return None
+        inv = np.argsort(bp.order)
+        status = sp["status"].cpu().numpy()[inv]
+        power = bp.unpack(sp["pw"].cpu().numpy())
+        power_raw = bp.unpack(sp["pr"].cpu().numpy())
+        sde = sp["sde"].cpu().numpy()[inv].astype(object)
+        sde_raw = sp["sde_raw"].cpu().numpy()[inv].astype(object)
+        if np.any(status != 0):
+            c_all = bp.unpack(chi2.cpu().numpy())
+            for i in np.nonzero(status != 0)[0]:
+                _, pr, pw, sr, sd = SP.spectra(SP.mask_outliers(c_all[i]), oversampling_factor)
+                power[i], power_raw[i], sde[i], sde_raw[i] = pw, pr, sd, sr
+        return dict(power=power, power_raw=power_raw, SDE=sde, SDE_raw=sde_raw,
+                    device=status == 0)
+
     def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
         """Enqueue the batched search on torch's current stream (no synchronisation)."""
         import torch

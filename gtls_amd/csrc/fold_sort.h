@@ -46,6 +46,27 @@ __device__ __forceinline__ bool key_greater(double ka, int32_t ia, double kb, in
     return ka > kb || (ka == kb && ia > ib);
 }
 
+// Compare-exchange of two (phase, index) keys held in registers.
+__device__ __forceinline__ void cmp_swap(double& ka, int32_t& ia, double& kb, int32_t& ib) {
+    const bool sw = key_greater(ka, ia, kb, ib);
+    const double k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    const int32_t i0 = sw ? ib : ia, i1 = sw ? ia : ib;
+    ka = k0; kb = k1; ia = i0; ib = i1;
+}
+
+// Batcher's odd-even merge network for 8 keys (19 compare-exchanges); unused slots hold
+// +inf keys, which sort last.
+__device__ __forceinline__ void sort8(double* k, int32_t* id) {
+#define GTLS_CS(a, b) cmp_swap(k[a], id[a], k[b], id[b])
+    GTLS_CS(0, 1); GTLS_CS(2, 3); GTLS_CS(4, 5); GTLS_CS(6, 7);
+    GTLS_CS(0, 2); GTLS_CS(1, 3); GTLS_CS(4, 6); GTLS_CS(5, 7);
+    GTLS_CS(1, 2); GTLS_CS(5, 6);
+    GTLS_CS(0, 4); GTLS_CS(1, 5); GTLS_CS(2, 6); GTLS_CS(3, 7);
+    GTLS_CS(2, 4); GTLS_CS(3, 5);
+    GTLS_CS(1, 2); GTLS_CS(3, 4); GTLS_CS(5, 6);
+#undef GTLS_CS
+}
+
 // Sorts the n samples of t by (phase, index) for period P. Writes the permutation to
 // perm[0..n) and the sorted phases to key_out[0..n); key_in is n doubles of scratch.
 // hist is LDS of kMaxBuckets ints, red is LDS of blockDim/64 ints. All threads call it.
@@ -84,10 +105,33 @@ __device__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, 
         key_out[pos] = ph;
     }
     __syncthreads();
-    // hist[b] now holds the end of bucket b; order each bucket by (phase, index)
+    // hist[b] now holds the end of bucket b; order each bucket by (phase, index): up to 8
+    // keys in registers through a sorting network (independent loads, no dependent
+    // memory round trips), larger buckets by insertion in place
     for (int b = tid; b < B; b += nt) {
         const int lo = b ? hist[b - 1] : 0;
         const int hi = hist[b];
+        const int cnt = hi - lo;
+        if (cnt <= 1) continue;
+        if (cnt <= 8) {
+            double k[8];
+            int32_t id[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool in = e < cnt;
+                k[e] = in ? key_out[lo + (in ? e : 0)] : __builtin_inf();
+                id[e] = in ? perm[lo + (in ? e : 0)] : 0x7fffffff;
+            }
+            sort8(k, id);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (e < cnt) {
+                    key_out[lo + e] = k[e];
+                    perm[lo + e] = id[e];
+                }
+            }
+            continue;
+        }
         for (int i = lo + 1; i < hi; ++i) {
             const double k = key_out[i];
             const int32_t id = perm[i];

@@ -52,26 +52,35 @@ int hip_fail(gtls_ctx* ctx, hipError_t e, const char* where) {
 }
 
 struct SearchLayout {
-    int64_t uw0 = 0, tmpl2 = 0, flat = 0, queue = 0, slots = 0;
+    int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, ss = 0, wconst = 0, flat = 0, queue = 0, slots = 0;
     int64_t slot_bytes = 0, o_key_in = 0, o_key_out = 0, o_perm = 0, o_vw = 0, o_D = 0;
     int64_t n_slots = 0, total = 0;
 };
 
 SearchLayout plan_search(const gtls_ctx* ctx, int mode, int64_t total_points,
-                         int64_t total_tmpl, int32_t n_curves, int64_t max_points,
-                         int64_t max_width) {
+                         int64_t total_tmpl, int64_t total_widths, int32_t n_curves,
+                         int64_t max_points, int64_t max_width) {
     const int64_t real = (mode == GTLS_MODE_FP64) ? 8 : 4;
     SearchLayout L;
     int64_t off = 0;
     L.uw0 = off;
     off = align_up(off + 2 * real * total_points);
+    const int64_t tmpl_padded = total_tmpl + 2 * kTmplPad * total_widths;
     L.tmpl2 = off;
-    off = align_up(off + 2 * real * (total_tmpl > 0 ? total_tmpl : 1));
+    off = align_up(off + 2 * real * tmpl_padded);
+    L.tmpl1 = off;
+    off = align_up(off + real * tmpl_padded);
+    L.ss = off;
+    off = align_up(off + 8 * (total_widths > 0 ? total_widths : 1));
+    L.wconst = off;
+    off = align_up(off + 8 * (int64_t)n_curves);
     L.flat = off;
     off = align_up(off + 8 * (int64_t)n_curves);
     L.queue = off;
     off = align_up(off + 8);
-    const int64_t np_ = max_points + (max_width > 0 ? max_width : 1);
+    // gathered samples: N + widest width of the wrap, plus up to another max_width read
+    // (and discarded) by the last lane of a register-blocked window
+    const int64_t np_ = max_points + 2 * (max_width > 0 ? max_width : 1) + 64;
     int64_t s = 0;
     L.o_key_in = s;
     s = align_up(s + 8 * max_points);
@@ -119,18 +128,21 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     hipStream_t st = ctx->stream;
     R2* uw0 = (R2*)(ws + L.uw0);
     R2* tmpl2 = (R2*)(ws + L.tmpl2);
+    Real* tmpl1 = (Real*)(ws + L.tmpl1);
+    double* ss = (double*)(ws + L.ss);
+    double* wconst = (double*)(ws + L.wconst);
     double* flat = (double*)(ws + L.flat);
     unsigned long long* queue = (unsigned long long*)(ws + L.queue);
 
     const int tpb = 256;
     hipLaunchKernelGGL(prep_points_kernel<Real>, dim3((unsigned)((b->total_points + tpb - 1) / tpb)),
                        dim3(tpb), 0, st, b->y, b->dy, b->total_points, uw0);
-    if (b->total_tmpl > 0)
-        hipLaunchKernelGGL(prep_template_kernel<Real>,
-                           dim3((unsigned)((b->total_tmpl + tpb - 1) / tpb)), dim3(tpb), 0, st,
-                           b->tmpl, b->total_tmpl, tmpl2);
+    hipLaunchKernelGGL(prep_template_kernel<Real>, dim3((unsigned)b->total_widths), dim3(tpb), 0,
+                       st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, tmpl2, tmpl1);
+    hipLaunchKernelGGL(template_ss_kernel<Real>, dim3((unsigned)b->total_widths), dim3(kWave), 0,
+                       st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, ss);
     hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
-                       b->y, b->dy, flat);
+                       b->y, b->dy, flat, wconst);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, st, queue);
 
     SearchParams<Real> p;
@@ -147,7 +159,10 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.tmpl_off = b->tmpl_off;
     p.uw0 = uw0;
     p.tmpl2 = tmpl2;
+    p.tmpl1 = tmpl1;
+    p.ss = ss;
     p.chi2_flat = flat;
+    p.wconst = wconst;
     p.skip_point = b->skip_point;
     p.depth_min = b->depth_min;
     p.negative_time = b->negative_time;
@@ -226,8 +241,8 @@ int64_t gtls_search_workspace_bytes(gtls_ctx* ctx, int32_t mode, const gtls_batc
     if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C) return fail(ctx, GTLS_E_MODE, "unknown mode");
     std::string why;
     if (!batch_ok(b, why)) return fail(ctx, GTLS_E_SHAPE, why);
-    return plan_search(ctx, mode, b->total_points, b->total_tmpl, b->n_curves, b->max_points,
-                       b->max_width).total;
+    return plan_search(ctx, mode, b->total_points, b->total_tmpl, b->total_widths,
+                       b->n_curves, b->max_points, b->max_width).total;
 }
 
 int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* b, void* workspace,
@@ -239,8 +254,9 @@ int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* b, void* 
     if (!batch_ok(b, why)) return fail(ctx, GTLS_E_SHAPE, why);
     if (!workspace || !chi2 || !best_k || !best_s) return fail(ctx, GTLS_E_SHAPE, "NULL output or workspace");
     DeviceGuard guard(ctx->device);
-    const SearchLayout L = plan_search(ctx, mode, b->total_points, b->total_tmpl, b->n_curves,
-                                       b->max_points, b->max_width);
+    const SearchLayout L = plan_search(ctx, mode, b->total_points, b->total_tmpl,
+                                       b->total_widths, b->n_curves, b->max_points,
+                                       b->max_width);
     if (workspace_bytes < L.total)
         return fail(ctx, GTLS_E_WORKSPACE, "workspace too small: need " + std::to_string(L.total) +
                                                " bytes, got " + std::to_string(workspace_bytes));
@@ -268,7 +284,7 @@ int64_t gtls_search_workspace_bytes_1(gtls_ctx* ctx, int32_t mode, int64_t n,
     if (n <= 0 || n_periods < 0 || n_widths <= 0 || total_tmpl < 0 || max_width < 0 ||
         n > (1LL << 30))
         return fail(ctx, GTLS_E_SHAPE, "sizes out of range");
-    return kSingleHead + plan_search(ctx, mode, n, total_tmpl, 1, n, max_width).total;
+    return kSingleHead + plan_search(ctx, mode, n, total_tmpl, n_widths, 1, n, max_width).total;
 }
 
 int gtls_search(gtls_ctx* ctx, int32_t mode, const double* t, const double* y,

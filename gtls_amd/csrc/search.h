@@ -18,6 +18,7 @@
 //   4. first minimum in (width, position) order (cp.argmin, core.py:866-875).
 #pragma once
 #include "fold_sort.h"
+#include "windows.h"
 
 namespace gtls {
 
@@ -44,7 +45,10 @@ struct SearchParams {
     const int64_t* tmpl_off;
     const R2* uw0;          // per point (u, w)
     const R2* tmpl2;        // per template sample (S, S^2)
+    const Real* tmpl1;      // per template sample S
+    const double* ss;       // per width: sum of S^2 over the row
     const double* chi2_flat;  // per curve
+    const double* wconst;   // per curve: the common weight 1/dy^2, or NaN if dy varies
     int64_t skip_point;
     double depth_min;
     int32_t negative_time;
@@ -133,7 +137,11 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
         wg_fold_sort(p.t + pt0, n, P, p.negative_time != 0, key_in, key_out, perm, hist,
                      red_i);
 
-        // 2. gather in phase order (wrapping) and the float64 running sum of u
+        // 2. gather in phase order (wrapping) and the float64 running sum of u. With a
+        //    common weight only u is kept (X = u), otherwise the pairs (u w, w).
+        const double wc = p.wconst[c];
+        const bool constw = wc == wc;
+        Real* ub = (Real*)vw;
         const int64_t np_ = n + dmax;
         {
             const int lane = lane_id(), wave = wave_id();
@@ -152,10 +160,14 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
                         const R2 a = p.uw0[pt0 + perm[j]];
                         uu = a.x;
                         ww = a.y;
-                        R2 o;
-                        o.x = uu * ww;
-                        o.y = ww;
-                        vw[i] = o;
+                        if (constw) {
+                            ub[i] = uu;
+                        } else {
+                            R2 o;
+                            o.x = uu * ww;
+                            o.y = ww;
+                            vw[i] = o;
+                        }
                     }
                     const double s = wave_inclusive_scan((double)uu);
                     u[e] = s + run;
@@ -176,41 +188,81 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
                 }
                 carry = total;
             }
+            // finite values past the sequence: the stride-1 window path multiplies them
+            // by zero template padding
+            if (tid < kTmplPad) {
+                if (constw) ub[np_ + tid] = (Real)0;
+                else vw[np_ + tid] = R2{(Real)0, (Real)0};
+            }
         }
         __syncthreads();
 
-        // 3. windows of every width of the period
+        // 3. windows: wavefront w takes widths klo + w, klo + w + nwaves, ...; every
+        //    per-width value is made wave-uniform so that template reads are scalar loads
+        //    and the segment predicates are scalar branches
         const double chi2f = p.chi2_flat[c];
         const double sentinel = (double)n;
         ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
-        for (int32_t k = klo; k <= khi; ++k) {
-            const int32_t d = p.widths[w0 + k];
-            if (d <= 0) {  // empty window: mean 0/0, every position carries the sentinel
-                if (tid == 0) consider<Real>(best, sentinel, k, 0);
-                continue;
-            }
-            const int64_t q = (d > p.skip_point) ? (d / p.skip_point) : 1;
-            const double os2 = 2.0 * p.overshoot[w0 + k];
-            const R2* T = p.tmpl2 + p.tmpl_off[w0 + k];
-            if (q > 1 && n > 1 && tid == 0) consider<Real>(best, sentinel, k, 1);
-            const int64_t ncand = (n + q - 1) / q;
-            for (int64_t ci = tid; ci < ncand; ci += kBlock) {
-                const int64_t s = ci * q;
-                const double m = (D[s + d - 1] - (s ? D[s - 1] : 0.0)) / (double)d;
-                if (m > p.depth_min) {
-                    const Real r = (Real)(m * os2);
-                    Real A = 0, Bw = 0;
-                    const R2* x = vw + s;
-                    for (int32_t i = 0; i < d; ++i) {
-                        const R2 xv = x[i];
-                        const R2 tv = T[i];
-                        A = fma(tv.x, xv.x, A);
-                        Bw = fma(tv.y, xv.y, Bw);
+        {
+            const int lane = lane_id();
+            const int wave = uniform((int32_t)wave_id());
+            const int nwaves = kBlock / kWave;
+            for (int32_t k = klo + wave; k <= khi; k += nwaves) {
+                const int64_t wg = w0 + k;                      // global width index
+                const int32_t d = uniform(p.widths[wg]);
+                if (d <= 0) {  // empty window: mean 0/0, every position carries the sentinel
+                    if (lane == 0) consider<Real>(best, sentinel, k, 0);
+                    continue;
+                }
+                const int32_t q = (int32_t)((d > p.skip_point) ? (d / p.skip_point) : 1);
+                const double os2 = 2.0 * p.overshoot[wg];
+                const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
+                if (q > 1 && n > 1 && lane == 0) consider<Real>(best, sentinel, k, 1);
+                const int64_t K = (n + q - 1) / q;
+                // candidates per lane: enough to cover K with one pass when possible, and
+                // (R - 1) q within the slot's padding of max_width samples
+                int32_t R = (int32_t)min((int64_t)kMaxBlock, max((int64_t)1, (K + kWave - 1) / kWave));
+                R = uniform(min(R, 1 + p.max_width / q));
+                const double ssk = p.ss[wg];
+                for (int64_t base = 0; base < K; base += (int64_t)kWave * R) {
+                    const int64_t k0 = base + (int64_t)lane * R;
+                    const int32_t rc = (int32_t)max((int64_t)0, min((int64_t)R, K - k0));
+                    Real A[kMaxBlock], B[kMaxBlock];
+#pragma unroll
+                    for (int r = 0; r < kMaxBlock; ++r) { A[r] = 0; B[r] = 0; }
+                    const int64_t s0 = (rc > 0 ? k0 : 0) * q;
+                    if (q == 1 && R == kMaxBlock) {
+                        if (constw)
+                            window_sums_q1<Real, true>(ub + s0, p.tmpl1 + toff, d, A, B);
+                        else
+                            window_sums_q1<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, A, B);
+                    } else {
+                        if (constw)
+                            window_sums_seg<Real, true>(ub + s0, p.tmpl1 + toff, d, q, R, A, B);
+                        else
+                            window_sums_seg<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, q, R, A, B);
                     }
-                    const Real delta = r * r * Bw - (Real)2 * r * A;
-                    consider<Real>(best, chi2f + (double)delta, k, (int32_t)s);
-                } else {
-                    consider<Real>(best, sentinel, k, (int32_t)s);
+#pragma unroll
+                    for (int r = 0; r < kMaxBlock; ++r) {
+                        if (r < rc) {
+                            const int64_t s = (k0 + r) * q;
+                            const double m = (D[s + d - 1] - (s ? D[s - 1] : 0.0)) / (double)d;
+                            if (m > p.depth_min) {
+                                const double rr = m * os2;
+                                double delta;
+                                if (constw) {
+                                    const Real rs = (Real)rr;
+                                    delta = wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)A[r]);
+                                } else {
+                                    const Real rs = (Real)rr;
+                                    delta = (double)(rs * rs * B[r] - (Real)2 * rs * A[r]);
+                                }
+                                consider<Real>(best, chi2f + delta, k, (int32_t)s);
+                            } else {
+                                consider<Real>(best, sentinel, k, (int32_t)s);
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -256,31 +308,67 @@ __global__ void prep_points_kernel(const double* y, const double* dy, int64_t n,
     uw[i] = o;
 }
 
+// Search-precision template rows, (S, S^2) pairs and S alone, each row surrounded by
+// kTmplPad zeros: row w starts at tmpl_off[w] + 2 kTmplPad w + kTmplPad. One block per row.
 template <class Real>
-__global__ void prep_template_kernel(const double* S, int64_t n, typename Pair<Real>::type* out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double s = S[i];
-    typename Pair<Real>::type o;
-    o.x = (Real)s;
-    o.y = (Real)(s * s);
-    out[i] = o;
+__global__ void prep_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
+                                     int64_t nw, typename Pair<Real>::type* out, Real* out1) {
+    const int64_t w = blockIdx.x;
+    if (w >= nw) return;
+    const int32_t d = max(widths[w], 0);
+    const int64_t src = off[w];
+    const int64_t dst = off[w] + 2 * kTmplPad * w;   // start of the padded row
+    for (int32_t i = threadIdx.x; i < d + 2 * kTmplPad; i += blockDim.x) {
+        const int32_t j = i - kTmplPad;
+        const double s = (j >= 0 && j < d) ? S[src + j] : 0.0;
+        typename Pair<Real>::type o;
+        o.x = (Real)s;
+        o.y = (Real)(s * s);
+        out[dst + i] = o;
+        out1[dst + i] = (Real)s;
+    }
 }
 
-// Flat-model chi2 of each curve, sum (1 - y)^2 / dy^2, float64; one workgroup per curve.
+// Sum of S^2 over each template row (the in-transit weight sum when all weights agree),
+// with S rounded to the search precision first; one wavefront per width.
+template <class Real>
+__global__ __launch_bounds__(kWave) void template_ss_kernel(const double* S, const int64_t* off,
+                                                           const int32_t* widths, int64_t nw,
+                                                           double* ss) {
+    const int64_t k = blockIdx.x;
+    if (k >= nw) return;
+    const int32_t d = widths[k];
+    double acc = 0.0;
+    for (int32_t i = threadIdx.x; i < d; i += kWave) {
+        const double s = (double)(Real)S[off[k] + i];
+        acc += s * s;
+    }
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) ss[k] = acc;
+}
+
+// Flat-model chi2 of each curve, sum (1 - y)^2 / dy^2, float64, and the curve's common
+// weight 1/dy^2 when every dy is the same value (NaN otherwise); one workgroup per curve.
 __global__ __launch_bounds__(kBlock) void chi2_flat_kernel(const int64_t* pt_off,
                                                            const double* y, const double* dy,
-                                                           double* out) {
+                                                           double* out, double* wconst) {
     __shared__ double red[kBlock / kWave];
+    __shared__ int differ;
     const int c = blockIdx.x;
     const int64_t a = pt_off[c], b = pt_off[c + 1];
+    if (threadIdx.x == 0) differ = 0;
+    __syncthreads();
+    const double e0 = dy[a];
     double s = 0.0;
+    int dif = 0;
     for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
         const double u = 1.0 - y[i];
         const double e = dy[i];
         const double w = 1.0 / (e * e);
         s += u * u * w;
+        dif |= (e != e0);
     }
+    if (dif) atomicOr(&differ, 1);
     s = wave_sum(s);
     if (lane_id() == 0) red[wave_id()] = s;
     __syncthreads();
@@ -288,6 +376,7 @@ __global__ __launch_bounds__(kBlock) void chi2_flat_kernel(const int64_t* pt_off
         double tot = 0.0;
         for (int w = 0; w < kBlock / kWave; ++w) tot += red[w];
         out[c] = tot;
+        wconst[c] = differ ? __builtin_nan("") : 1.0 / (e0 * e0);
     }
 }
 

@@ -10,9 +10,10 @@
 //   pass 2 (workgroup per curve): power = power_raw - median, minus its mean;
 //                                 SDE = max(power / std); power *= SDE / max(power)
 //
-// Curves whose chi2 holds a NaN, or whose largest chi2 exceeds 100x its smallest (the only
+// Curves whose chi2 holds a NaN, whose largest chi2 exceeds 100x its smallest (the only
 // case in which the reference's outlier mask, src/gputls/core.py:890-896, can be
-// non-empty), get status 1 and are left to the host.
+// non-empty), or whose SR or detrended power has zero spread (NumPy's masked division
+// then masks the result) get status 1 and are left to the host.
 #pragma once
 #include "wg_primitives.h"
 
@@ -90,8 +91,12 @@ __global__ __launch_bounds__(kSpecBlock) void spectra_pass1_kernel(SpecParams p)
         v += d * d;
         mx = fmax(mx, d);
     }
-    const double sd = sqrt(block_reduce_sum(v, tmp) / (double)n);
-    mx = block_reduce_max(mx, tmp);
+    const double sd = uniform(sqrt(block_reduce_sum(v, tmp) / (double)n));
+    mx = uniform(block_reduce_max(mx, tmp));
+    if (!(sd > 0.0) || !(mx > 0.0)) {  // zero spread: the masked-array result is the host's
+        if (tid == 0) p.status[c] = 1;
+        return;
+    }
     const double sde_raw = (1.0 - mean) / sd;
     const double scale = sde_raw / mx;
     const bool detrend = n > 2 * (int64_t)p.kernel;
@@ -177,8 +182,12 @@ __global__ __launch_bounds__(kSpecBlock) void spectra_pass2_kernel(SpecParams p,
         v += d * d;
         mx = fmax(mx, q);
     }
-    const double sd = sqrt(block_reduce_sum(v, tmp) / (double)n);
-    mx = block_reduce_max(mx, tmp);
+    const double sd = uniform(sqrt(block_reduce_sum(v, tmp) / (double)n));
+    mx = uniform(block_reduce_max(mx, tmp));
+    if (!(sd > 0.0) || !(mx > 0.0)) {
+        if (tid == 0) p.status[c] = 1;
+        return;
+    }
     const double sde = mx / sd;
     const double scale = sde / mx;
     for (int64_t i = a + tid; i < b; i += blockDim.x)

@@ -1,0 +1,78 @@
+"""Batched search (one persistent launch over many curves) and device spectra, through
+the C ABI, against the FP64 oracle per curve and the host spectra."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from gtls_amd import _ffi
+from gtls_amd import batch as B
+from gtls_amd import search as S
+from gtls_amd import spectra as SP
+from gtls_amd import synthetic
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(n_curves=12, seed=9):
+    return synthetic.batch(n_curves, seed=seed, n_min=300, n_max=6000)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32c"])
+def test_batched_matches_oracle_per_curve(gpu, mode):
+    curves = small_batch()
+    plans = B.build_plans(curves, {}, workers=1)
+    bp = B.BatchPlan(plans)
+    db = B.DeviceBatch(bp)
+    db.search(mode, 8)
+    chi2 = bp.unpack(db.chi2.cpu().numpy())
+    bk = bp.unpack(db.best_k.cpu().numpy())
+    bs = bp.unpack(db.best_s.cpu().numpy())
+    tol = 1e-9 if mode == "fp64" else 2e-6
+    worst = 0.0
+    for i, p in enumerate(plans):
+        co, ko, so = oracle.search_plan(p, skip_point=8)
+        rel = np.max(np.abs(chi2[i] - co) / np.abs(co))
+        worst = max(worst, rel)
+        assert rel < tol, (i, rel)
+        if mode == "fp64":
+            assert np.array_equal(bk[i], ko) and np.array_equal(bs[i], so), i
+        else:
+            assert np.mean((bk[i] == ko) & (bs[i] == so)) > 0.98
+    print("batched %s: %d curves, %d periods, max rel chi2 err %.3g"
+          % (mode, len(plans), bp.total_periods, worst))
+
+
+def test_batched_equals_single_curve_launches(gpu):
+    """The same curve gives the same bits in the batch and alone."""
+    curves = small_batch(6, seed=3)
+    plans = B.build_plans(curves, {}, workers=1)
+    bp = B.BatchPlan(plans)
+    db = B.DeviceBatch(bp)
+    db.search("fp32c", 8)
+    chi2 = bp.unpack(db.chi2.cpu().numpy())
+    for i, p in enumerate(plans):
+        c, k, s = S.DeviceCurve(p).search(p.periods, p.k_lo, p.k_hi, mode="fp32c")
+        assert np.array_equal(c.cpu().numpy(), chi2[i])
+
+
+def test_device_spectra_match_host(gpu):
+    curves = small_batch(10, seed=5)
+    plans = B.build_plans(curves, {}, workers=1)
+    bp = B.BatchPlan(plans)
+    db = B.DeviceBatch(bp)
+    chi2, _, _ = db.search("fp64", 8)
+    res = db.spectra(chi2, 3)
+    c_all = bp.unpack(chi2.cpu().numpy())
+    assert res["device"].sum() >= bp.n_curves // 2
+    for i in range(bp.n_curves):
+        _, p_raw, p_det, s_raw, s_det = SP.spectra(SP.mask_outliers(c_all[i]), 3)
+        if not res["device"][i]:   # host path: identical by construction
+            continue
+        assert np.allclose(res["power_raw"][i], np.asarray(p_raw), rtol=1e-12, atol=1e-12)
+        assert np.allclose(res["power"][i], np.asarray(p_det), rtol=1e-10, atol=1e-10)
+        assert abs(res["SDE_raw"][i] - s_raw) <= 1e-12 * abs(s_raw)
+        assert abs(res["SDE"][i] - s_det) <= 1e-10 * abs(s_det)
+        assert int(np.argmax(res["power"][i])) == int(np.argmax(np.asarray(p_det)))
+    print("device spectra on %d of %d curves" % (res["device"].sum(), bp.n_curves))

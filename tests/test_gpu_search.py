@@ -134,3 +134,16 @@ def test_edge_cases(gpu):
     c, k, s = [x.cpu().numpy() for x in dc.search(flat.periods[:5], flat.k_lo[:5],
                                                   flat.k_hi[:5])]
     assert np.all(c == t.size) and np.all(k == flat.k_lo[:5]) and np.all(s == 0)
+
+
+@pytest.mark.parametrize("skip", [2, 3, 5, 8, 16, 100])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_search_parity_skip_strides(gpu, skip, weighted):
+    """Every T0 stride (register-blocked windows: segment and remainder bookkeeping) on
+    both paths: common weight (dy constant) and per-sample weights."""
+    t, y = small_curve(1500, 11)
+    dy = np.random.default_rng(4).uniform(4e-4, 9e-4, t.size) if weighted else None
+    plan = plan_for(t, y, dy)
+    sel = np.arange(0, len(plan.periods), 3)
+    for mode in ("fp64", "fp32c"):
+        assert_parity(plan, mode, skip, plan.periods[sel], plan.k_lo[sel], plan.k_hi[sel])
