@@ -67,6 +67,48 @@ __device__ __forceinline__ void sort8(double* k, int32_t* id) {
 #undef GTLS_CS
 }
 
+// Batcher's odd-even merge network for 16 keys (63 compare-exchanges).
+__device__ __forceinline__ void sort16(double* k, int32_t* id) {
+#define GTLS_CS(a, b) cmp_swap(k[a], id[a], k[b], id[b])
+    GTLS_CS(0, 1); GTLS_CS(2, 3); GTLS_CS(0, 2); GTLS_CS(1, 3); GTLS_CS(1, 2);
+    GTLS_CS(4, 5); GTLS_CS(6, 7); GTLS_CS(4, 6); GTLS_CS(5, 7); GTLS_CS(5, 6);
+    GTLS_CS(0, 4); GTLS_CS(2, 6); GTLS_CS(2, 4); GTLS_CS(1, 5); GTLS_CS(3, 7);
+    GTLS_CS(3, 5); GTLS_CS(1, 2); GTLS_CS(3, 4); GTLS_CS(5, 6); GTLS_CS(8, 9);
+    GTLS_CS(10, 11); GTLS_CS(8, 10); GTLS_CS(9, 11); GTLS_CS(9, 10); GTLS_CS(12, 13);
+    GTLS_CS(14, 15); GTLS_CS(12, 14); GTLS_CS(13, 15); GTLS_CS(13, 14); GTLS_CS(8, 12);
+    GTLS_CS(10, 14); GTLS_CS(10, 12); GTLS_CS(9, 13); GTLS_CS(11, 15); GTLS_CS(11, 13);
+    GTLS_CS(9, 10); GTLS_CS(11, 12); GTLS_CS(13, 14); GTLS_CS(0, 8); GTLS_CS(4, 12);
+    GTLS_CS(4, 8); GTLS_CS(2, 10); GTLS_CS(6, 14); GTLS_CS(6, 10); GTLS_CS(2, 4);
+    GTLS_CS(6, 8); GTLS_CS(10, 12); GTLS_CS(1, 9); GTLS_CS(5, 13); GTLS_CS(5, 9);
+    GTLS_CS(3, 11); GTLS_CS(7, 15); GTLS_CS(7, 11); GTLS_CS(3, 5); GTLS_CS(7, 9);
+    GTLS_CS(11, 13); GTLS_CS(1, 2); GTLS_CS(3, 4); GTLS_CS(5, 6); GTLS_CS(7, 8);
+    GTLS_CS(9, 10); GTLS_CS(11, 12); GTLS_CS(13, 14);
+#undef GTLS_CS
+}
+
+// Up to M keys of one bucket: load (independent loads), sort in registers, store.
+template <int M>
+__device__ __forceinline__ void sort_bucket_regs(double* __restrict__ key, int32_t* __restrict__ perm,
+                                                 int lo, int cnt) {
+    double k[M];
+    int32_t id[M];
+#pragma unroll
+    for (int e = 0; e < M; ++e) {
+        const bool in = e < cnt;
+        k[e] = in ? key[lo + (in ? e : 0)] : __builtin_inf();
+        id[e] = in ? perm[lo + (in ? e : 0)] : 0x7fffffff;
+    }
+    if constexpr (M == 8) sort8(k, id);
+    else sort16(k, id);
+#pragma unroll
+    for (int e = 0; e < M; ++e) {
+        if (e < cnt) {
+            key[lo + e] = k[e];
+            perm[lo + e] = id[e];
+        }
+    }
+}
+
 // Sorts the n samples of t by (phase, index) for period P. Writes the permutation to
 // perm[0..n) and the sorted phases to key_out[0..n); key_in is n doubles of scratch.
 // hist is LDS of kMaxBuckets ints, red is LDS of blockDim/64 ints. All threads call it.
@@ -105,7 +147,7 @@ __device__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, 
         key_out[pos] = ph;
     }
     __syncthreads();
-    // hist[b] now holds the end of bucket b; order each bucket by (phase, index): up to 8
+    // hist[b] now holds the end of bucket b; order each bucket by (phase, index): up to 16
     // keys in registers through a sorting network (independent loads, no dependent
     // memory round trips), larger buckets by insertion in place
     for (int b = tid; b < B; b += nt) {
@@ -114,22 +156,11 @@ __device__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, 
         const int cnt = hi - lo;
         if (cnt <= 1) continue;
         if (cnt <= 8) {
-            double k[8];
-            int32_t id[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const bool in = e < cnt;
-                k[e] = in ? key_out[lo + (in ? e : 0)] : __builtin_inf();
-                id[e] = in ? perm[lo + (in ? e : 0)] : 0x7fffffff;
-            }
-            sort8(k, id);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                if (e < cnt) {
-                    key_out[lo + e] = k[e];
-                    perm[lo + e] = id[e];
-                }
-            }
+            sort_bucket_regs<8>(key_out, perm, lo, cnt);
+            continue;
+        }
+        if (cnt <= 16) {
+            sort_bucket_regs<16>(key_out, perm, lo, cnt);
             continue;
         }
         for (int i = lo + 1; i < hi; ++i) {

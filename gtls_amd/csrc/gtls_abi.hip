@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -18,6 +19,7 @@ struct gtls_ctx {
     int slots_fp64 = 0;
     int slots_fp32 = 0;
     int slots_sort = 0;
+    int use_mfma = 1;
     std::string err;
 };
 
@@ -65,7 +67,9 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, int64_t total_points,
     int64_t off = 0;
     L.uw0 = off;
     off = align_up(off + 2 * real * total_points);
-    const int64_t tmpl_padded = total_tmpl + 2 * kTmplPad * total_widths;
+    // rows with kTmplPad zeros on both sides, then a zeroed tail for the over-reads of the
+    // matrix-core path (up to one block, i.e. max_width samples, past a row)
+    const int64_t tmpl_padded = total_tmpl + 2 * kTmplPad * total_widths + kTmplTail;
     L.tmpl2 = off;
     off = align_up(off + 2 * real * tmpl_padded);
     L.tmpl1 = off;
@@ -117,6 +121,10 @@ bool batch_ok(const gtls_batch* b, std::string& why) {
         why = "a light curve or width exceeds 2^30 samples";
         return false;
     }
+    if (b->max_width > kTmplTail - 64) {
+        why = "a template width exceeds the engine's limit of 32704 samples";
+        return false;
+    }
     if (b->skip_point <= 0) { why = "skip_point must be positive"; return false; }
     return true;
 }
@@ -137,8 +145,9 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     const int tpb = 256;
     hipLaunchKernelGGL(prep_points_kernel<Real>, dim3((unsigned)((b->total_points + tpb - 1) / tpb)),
                        dim3(tpb), 0, st, b->y, b->dy, b->total_points, uw0);
-    hipLaunchKernelGGL(prep_template_kernel<Real>, dim3((unsigned)b->total_widths), dim3(tpb), 0,
-                       st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, tmpl2, tmpl1);
+    hipLaunchKernelGGL(prep_template_kernel<Real>, dim3((unsigned)b->total_widths + 8), dim3(tpb),
+                       0, st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->total_tmpl, tmpl2,
+                       tmpl1);
     hipLaunchKernelGGL(template_ss_kernel<Real>, dim3((unsigned)b->total_widths), dim3(kWave), 0,
                        st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, ss);
     hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
@@ -167,6 +176,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.depth_min = b->depth_min;
     p.negative_time = b->negative_time;
     p.total_periods = b->total_periods;
+    p.use_mfma = ctx->use_mfma;
     p.max_points = b->max_points;
     p.max_width = b->max_width > 0 ? b->max_width : 1;
     p.queue = queue;
@@ -217,6 +227,7 @@ int gtls_create(int device, void* stream, gtls_ctx** out) {
     ctx->device = device;
     ctx->stream = (hipStream_t)stream;
     ctx->n_cu = prop.multiProcessorCount;
+    if (const char* e = std::getenv("GTLS_MFMA")) ctx->use_mfma = std::atoi(e) != 0;
     ctx->slots_fp64 = occupancy_slots(ctx, (const void*)tls_search_kernel<double>);
     ctx->slots_fp32 = occupancy_slots(ctx, (const void*)tls_search_kernel<float>);
     ctx->slots_sort = occupancy_slots(ctx, (const void*)fold_sort_kernel);

@@ -23,6 +23,7 @@
 namespace gtls {
 
 constexpr int kBlock = 256;
+constexpr int64_t kTmplTail = 32768;  // zeroed samples after the last template row
 constexpr int kScanPerThread = 8;  // gathered elements per thread per prefix-scan chunk
 
 template <class Real> struct Pair;
@@ -53,6 +54,7 @@ struct SearchParams {
     double depth_min;
     int32_t negative_time;
     int64_t total_periods;
+    int32_t use_mfma;     // float32: window sums on the matrix cores
     int32_t max_points;   // slot capacity: every curve must have n <= max_points
     int32_t max_width;    // and every searched width d <= max_width
     unsigned long long* queue;
@@ -88,6 +90,7 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
     __shared__ double red_d[kBlock / kWave];
     __shared__ ArgMin red_a[kBlock / kWave];
     __shared__ unsigned long long s_item;
+    __shared__ float rings[2][kBlock / kWave][kRingRows * kRingStride];
 
     const int tid = threadIdx.x;
     char* slot = p.slots + (int64_t)blockIdx.x * p.slot_bytes;
@@ -224,6 +227,35 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
                 int32_t R = (int32_t)min((int64_t)kMaxBlock, max((int64_t)1, (K + kWave - 1) / kWave));
                 R = uniform(min(R, 1 + p.max_width / q));
                 const double ssk = p.ss[wg];
+                const int32_t F = d / q;
+                if constexpr (sizeof(Real) == 4) {
+                    if (p.use_mfma && q >= 4 && F + 1 <= 16) {
+                        // candidate k gets its window sums from the matrix-core tiles
+                        auto emit = [&](int64_t kc, float sa, float sb, double dlo, double dhi) {
+                            const int64_t s = kc * q;
+                            const double m = (dhi - dlo) / (double)d;
+                            if (m > p.depth_min) {
+                                const float rs = (float)(m * os2);
+                                const double delta =
+                                    constw ? wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)sa)
+                                           : (double)(rs * rs * sb - 2.f * rs * sa);
+                                consider<Real>(best, chi2f + delta, k, (int32_t)s);
+                            } else {
+                                consider<Real>(best, sentinel, k, (int32_t)s);
+                            }
+                        };
+                        float* ra = rings[0][wave];
+                        float* rb = rings[1][wave];
+                        if (constw)
+                            window_sums_mfma<true>((const float*)ub, np_, (const float*)(p.tmpl1 + toff),
+                                                   D, d, q, K, ra, rb, emit);
+                        else
+                            window_sums_mfma<false>((const float*)vw, np_,
+                                                    (const float*)(p.tmpl2 + toff), D, d, q, K, ra,
+                                                    rb, emit);
+                        continue;
+                    }
+                }
                 for (int64_t base = 0; base < K; base += (int64_t)kWave * R) {
                     const int64_t k0 = base + (int64_t)lane * R;
                     const int32_t rc = (int32_t)max((int64_t)0, min((int64_t)R, K - k0));
@@ -312,9 +344,22 @@ __global__ void prep_points_kernel(const double* y, const double* dy, int64_t n,
 // kTmplPad zeros: row w starts at tmpl_off[w] + 2 kTmplPad w + kTmplPad. One block per row.
 template <class Real>
 __global__ void prep_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
-                                     int64_t nw, typename Pair<Real>::type* out, Real* out1) {
+                                     int64_t nw, int64_t total_tmpl,
+                                     typename Pair<Real>::type* out, Real* out1) {
     const int64_t w = blockIdx.x;
-    if (w >= nw) return;
+    if (w >= nw) {
+        // the zeroed tail after the last row (over-reads of the matrix-core path)
+        const int64_t end = total_tmpl + 2 * kTmplPad * nw;
+        for (int64_t i = end + (int64_t)(w - nw) * blockDim.x + threadIdx.x;
+             i < end + kTmplTail; i += (int64_t)(gridDim.x - nw) * blockDim.x) {
+            typename Pair<Real>::type o;
+            o.x = (Real)0;
+            o.y = (Real)0;
+            out[i] = o;
+            out1[i] = (Real)0;
+        }
+        return;
+    }
     const int32_t d = max(widths[w], 0);
     const int64_t src = off[w];
     const int64_t dst = off[w] + 2 * kTmplPad * w;   // start of the padded row
