@@ -3,18 +3,21 @@
 Workload (synthetic, seeded): 4096 light curves, N log-uniform in [2000, 70000] points,
 cadence 2 or 30 min, noise U(1e-4, 2e-3), half with an injected limb-darkened transit,
 each curve with its own Ofir period grid, duration grid and template table (default TLS
-settings, T0_fit_margin 0.125). A step is one pass of the search over the whole batch:
-the persistent chi2-vs-period search kernel over every (curve, period) plus the device
-SR/SDE spectra of every curve. Inputs are resident in HBM before the timed region.
+settings, T0_fit_margin 0.125). The set is cut into batches of --batch curves (512 by
+default); a step is one pass of the search over one batch: the persistent
+chi2-vs-period search kernel over every (curve, period) of the batch plus the device
+SR/SDE spectra of its curves. Step i searches batch i mod (curves / batch), so the default
+8 steps cover the whole 4096-curve set once. Inputs of every batch are resident in HBM
+before the timed region.
 
 Also measured in the same run (reported beside `value`, never as `value`):
   plain    -- the float64 search one curve per launch with host spectra, on a stratified
-              sample of curves, extrapolated to the batch by work (sum N' x periods x widths)
+              sample of curves, extrapolated to the set by window work
   parity   -- on that sample: same best period (SDE peak) and same best width, SDE error
   cpu      -- the FP64 CPU oracle (OpenMP, all host cores) on a bounded sample of curves,
-              extrapolated the same way
+              extrapolated the same way, with the same peak/SDE check
 
-    python bench.py [--steps K] [--warmup W] [--curves 4096]
+    python bench.py [--steps K] [--warmup W] [--curves 4096] [--batch 512]
 """
 from __future__ import annotations
 
@@ -29,6 +32,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+FP32_MATRIX_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix-core peak
 HBM_PEAK_GBS = 8000.0
 
 
@@ -36,74 +40,106 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def curve_work(plan):
-    """sum over periods of N' * (number of searched widths): the search's work unit."""
+def window_macs(plan, skip):
+    """Useful multiply-accumulates of the window sums of one curve: for every period and
+    every searched width d (stride q = max(d // skip, 1)), ceil(N / q) positions of d
+    template samples each (a second product per sample for per-sample weights is not
+    counted)."""
+    w = plan.table.widths.astype(np.int64)
+    q = np.maximum(w // skip, 1) if skip < (1 << 30) else np.ones_like(w)
+    per_width = -(-plan.n // q) * np.maximum(w, 0)
+    c = np.concatenate([[0], np.cumsum(per_width)])
+    ok = plan.k_lo <= plan.k_hi
+    return float(np.sum((c[plan.k_hi + 1] - c[plan.k_lo])[ok]))
+
+
+def algorithmic_bytes(plan, real_bytes=4):
+    """Algorithmic HBM bytes of one curve's search: per period the times (8 B) read for the
+    fold, and per searched sample the gathered flux deviation and weight
+    (2 x real_bytes) of the N' = N + widest-width samples, plus 16 B of result."""
     ok = plan.k_lo <= plan.k_hi
     wmax = plan.table.widths[np.clip(plan.k_hi, 0, None)]
-    nw = np.where(ok, plan.k_hi - plan.k_lo + 1, 0)
-    return float(np.sum((plan.n + wmax) * nw * ok))
+    return float(np.sum(((8 + 2 * real_bytes) * plan.n + 2 * real_bytes * wmax + 16) * ok))
 
 
-def algorithmic_bytes(plans, real_bytes=4):
-    """Algorithmic HBM bytes of one search launch: per (curve, period) the permutation
-    (4 B) and the gathered flux deviation and weight (2 x real_bytes) for each of the
-    N' = N + widest-width samples, and 12 B of result per searched width."""
-    total = 0.0
-    for p in plans:
-        ok = p.k_lo <= p.k_hi
-        wmax = p.table.widths[np.clip(p.k_hi, 0, None)]
-        nw = np.where(ok, p.k_hi - p.k_lo + 1, 0)
-        total += float(np.sum(((4 + 2 * real_bytes) * (p.n + wmax) + 12 * nw) * ok))
-    return total
+def stratified(work, k):
+    """k curve indices spread evenly over the sorted work distribution."""
+    order = np.argsort(work, kind="stable")
+    idx = order[np.linspace(0, len(order) - 1, min(k, len(order))).astype(int)]
+    return list(dict.fromkeys(int(i) for i in idx))
 
 
-def stratified(plans, k, seed=0):
-    """k curves spread over the work distribution (sorted by work, evenly spaced)."""
-    w = np.array([curve_work(p) for p in plans])
-    order = np.argsort(w)
-    idx = order[np.linspace(0, len(order) - 1, k).astype(int)]
-    return sorted(set(int(i) for i in idx))
+def peak_check(sde_a, pw_a, bk_a, c_b, k_b, oversampling=3):
+    """Best period / best width / SDE of a search (c_b, k_b) against the batched result."""
+    from gtls_amd import spectra as SP
+    _, _, pw, _, sd = SP.spectra(SP.mask_outliers(np.asarray(c_b, np.float64)), oversampling)
+    ia, ib = int(np.argmax(np.asarray(pw_a))), int(np.argmax(np.asarray(pw)))
+    same_p = ia == ib
+    same_w = same_p and int(bk_a[ia]) == int(np.asarray(k_b)[ib])
+    return same_p, same_w, abs(float(sde_a) - float(sd)) / abs(float(sd))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--curves", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the TLS search of a batch runs on a single device "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build searches a batch on a single device "
+              "and reports no multi-GPU figure; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.curves % args.batch:
+        print("--curves must be a multiple of --batch", file=sys.stderr)
         sys.exit(2)
 
     import torch
 
     from gtls_amd import batch as B
-    from gtls_amd import synthetic
     from gtls_amd import search as S
-    from gtls_amd import spectra as SP
+    from gtls_amd import synthetic
     from gtls_amd.grid import skip_factor
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        # a plain copy of 2^28 float32 (1 GiB read, 1 GiB written), three launches
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30,
+                          "launches": 3}), flush=True)
+        return
+    skip = skip_factor(0.125)
 
     t0 = time.time()
     curves = synthetic.batch(args.curves, seed=args.seed)
     plans = B.build_plans(curves, {})
-    bp = B.BatchPlan(plans)
-    log("built %d plans: %d points, %d periods, %.1f s" % (
-        len(plans), int(bp.pt_off[-1]), bp.total_periods, time.time() - t0))
-    db = B.DeviceBatch(bp, 0)
-    skip = skip_factor(0.125)
+    del curves
+    n_batches = args.curves // args.batch
+    batches = []
+    for b in range(n_batches):
+        bp = B.BatchPlan(plans[b * args.batch:(b + 1) * args.batch])
+        batches.append((bp, B.DeviceBatch(bp, 0)))
+    macs = np.array([window_macs(p, skip) for p in plans])
+    algo = np.array([algorithmic_bytes(p, 4) for p in plans])
+    log("built %d plans in %d batches: %d points, %d periods, %.3g window MACs, %.1f s" % (
+        len(plans), n_batches, sum(p.n for p in plans), sum(len(p.periods) for p in plans),
+        macs.sum(), time.time() - t0))
 
-    def step(ev=None):
+    def step(i, ev=None):
+        bp, db = batches[i % n_batches]
         if ev is not None:
             ev[0].record()
         chi2, _, _ = db.search("fp32c", skip)
@@ -112,7 +148,7 @@ def main():
         db.spectra(chi2, 3, sync=False)
 
     for i in range(args.warmup):
-        step()
+        step(i)
         torch.cuda.synchronize()
         log("warmup step %d done" % i)
 
@@ -121,83 +157,116 @@ def main():
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):
-        step(evs[i])
-    torch.cuda.synchronize()
+        step(i, evs[i])
+        torch.cuda.synchronize()          # a progress line per finished step
+        log("step %d done" % i)
     elapsed = time.perf_counter() - t_start
-    search_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    search_ms = np.array([a.elapsed_time(b) for a, b in evs])
     ms_per_step = elapsed * 1e3 / args.steps
-    value = args.curves * args.steps / elapsed * 3600.0
-    log("timed %d steps: %.1f ms/step, search kernel %.1f ms" % (args.steps, ms_per_step,
-                                                                search_ms))
+    n_done = args.batch * args.steps
+    value = n_done / elapsed * 3600.0
+    log("timed %d steps: %.1f ms/step, search launches %.1f ms on average" % (
+        args.steps, ms_per_step, search_ms.mean()))
 
-    algo = algorithmic_bytes(plans, 4)
-    achieved = algo / (search_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "tls_search_kernel<float>", "algorithmic_bytes": algo}
+    # per step: the useful window FLOPs (2 per MAC) and algorithmic bytes of its batch
+    step_curves = [np.arange((i % n_batches) * args.batch, (i % n_batches + 1) * args.batch)
+                   for i in range(args.steps)]
+    flops = np.array([2.0 * macs[c].sum() for c in step_curves])
+    bytes_ = np.array([algo[c].sum() for c in step_curves])
+    achieved = float(np.mean(flops / (search_ms * 1e-3)) / 1e12)
+    hbm_achieved = float(np.mean(bytes_ / (search_ms * 1e-3)) / 1e9)
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3),
+                "peak": FP32_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_MATRIX_PEAK_TFLOPS, 5), "traffic": None,
+                "kernel": "tls_search_kernel<float>",
+                "launch_ms": round(float(search_ms.mean()), 3),
+                "launch_ms_per_step": [round(float(x), 3) for x in search_ms],
+                "algorithmic_flops_per_launch": float(flops.mean()),
+                "algorithmic_bytes_per_launch": float(bytes_.mean()),
+                "algorithmic_hbm_GBps": round(hbm_achieved, 1),
+                "algorithmic_hbm_frac": round(hbm_achieved / HBM_PEAK_GBS, 5)}
 
-    bk_b = bp.unpack(db.best_k.cpu().numpy())
-    spec = db.spectra(db.chi2, 3)          # finishes host fallbacks of the last step
-    pw_b, sde_b = spec["power"], spec["SDE"]
-    extra = {"device_spectra_curves": int(np.sum(spec["device"]))}
-    npd = bp.total_periods
+    # batched results of every searched batch (the last step of each batch wrote them)
+    res = {}
+    for b, (bp, db) in enumerate(batches[:min(args.steps, n_batches)]):
+        spec = db.spectra(db.chi2, 3)
+        bk = bp.unpack(db.best_k.cpu().numpy())
+        for j in range(args.batch):
+            ci = b * args.batch + j
+            res[ci] = (spec["SDE"][j], spec["power"][j], bk[j])
+    extra = {"batch_curves": args.batch, "batches_in_set": n_batches}
+    done = np.array(sorted(res))
+    total_macs = macs[done].sum()
 
-    total_work = sum(curve_work(p) for p in plans)
-
-    # plain float64 search, one curve per launch, host spectra; parity with the batch
     if not args.no_plain:
-        sample = stratified(plans, min(args.plain_sample, len(plans)))
-        t_plain, w_plain = 0.0, 0.0
-        same_period = same_width = 0
+        sample = [int(done[i]) for i in stratified(macs[done], args.plain_sample)]
+        t_plain = w_plain = 0.0
+        same_p = same_w = 0
         max_dsde = 0.0
         for ci in sample:
             p = plans[ci]
             dc = S.DeviceCurve(p, 0)
             torch.cuda.synchronize()
             ts = time.perf_counter()
-            c, k, s = dc.search(p.periods, p.k_lo, p.k_hi, mode="fp64", skip_point=skip)
-            c = c.cpu().numpy()
-            _, _, pw, _, sd = SP.spectra(SP.mask_outliers(c), 3)
+            c, k, _ = dc.search(p.periods, p.k_lo, p.k_hi, mode="fp64", skip_point=skip)
+            c, k = c.cpu().numpy(), k.cpu().numpy()
+            from gtls_amd import spectra as SP
+            SP.spectra(SP.mask_outliers(c), 3)
             t_plain += time.perf_counter() - ts
-            w_plain += curve_work(p)
-            kk = k.cpu().numpy()
-            ib, ip = int(np.argmax(np.asarray(pw_b[ci]))), int(np.argmax(np.asarray(pw)))
-            same_period += ib == ip
-            same_width += (ib == ip) and bk_b[ci][ib] == kk[ip]
-            max_dsde = max(max_dsde, abs(float(sde_b[ci]) - float(sd)) / abs(float(sd)))
-        plain_value = len(plans) / (t_plain * total_work / w_plain) * 3600.0
+            w_plain += macs[ci]
+            sp, sw, d = peak_check(*res[ci], c, k)
+            same_p += sp
+            same_w += sw
+            max_dsde = max(max_dsde, d)
+        plain_value = len(done) / (t_plain * total_macs / w_plain) * 3600.0
         extra.update({"plain_fp64_curves_per_hour": round(plain_value, 1),
                       "speedup_vs_plain_fp64": round(value / plain_value, 3),
                       "plain_sample_curves": len(sample),
-                      "parity_same_best_period": "%d/%d" % (same_period, len(sample)),
-                      "parity_same_best_width": "%d/%d" % (same_width, len(sample)),
-                      "parity_max_rel_sde_err": float(max_dsde)})
-        log("plain fp64: %.1f curves/h (sample %d), parity %s %s, max dSDE %.2e" % (
-            plain_value, len(sample), extra["parity_same_best_period"],
-            extra["parity_same_best_width"], max_dsde))
+                      "parity_vs_plain_same_best_period": "%d/%d" % (same_p, len(sample)),
+                      "parity_vs_plain_same_best_width": "%d/%d" % (same_w, len(sample)),
+                      "parity_vs_plain_max_rel_sde_err": float(max_dsde)})
+        log("plain fp64: %.1f curves/h (sample %d), parity %d/%d %d/%d, max dSDE %.2e" % (
+            plain_value, len(sample), same_p, len(sample), same_w, len(sample), max_dsde))
 
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
         cores = os.cpu_count() or 1
-        order = stratified(plans, min(64, len(plans)))
-        t_cpu, w_cpu, used = 0.0, 0.0, 0
-        for ci in order:
+        cand = [int(done[i]) for i in stratified(macs[done], 64)]
+        # smallest first to learn the rate, then the spread sample within the budget
+        first = min(cand, key=lambda i: macs[i])
+        t_cpu = w_cpu = 0.0
+        used = []
+        same_p = same_w = 0
+        max_dsde = 0.0
+        for ci in [first] + [i for i in cand if i != first]:
+            if used:
+                rate = t_cpu / w_cpu
+                if t_cpu + rate * macs[ci] > args.cpu_budget:
+                    continue
             p = plans[ci]
-            if t_cpu > args.cpu_budget:
-                break
             ts = time.perf_counter()
-            oracle.search_plan(p, skip_point=skip, nthreads=cores)
+            c, k, _ = oracle.search_plan(p, skip_point=skip, nthreads=cores)
             t_cpu += time.perf_counter() - ts
-            w_cpu += curve_work(p)
-            used += 1
-        cpu_value = len(plans) / (t_cpu * total_work / w_cpu) * 3600.0
+            w_cpu += macs[ci]
+            used.append(ci)
+            sp, sw, d = peak_check(*res[ci], c, k)
+            same_p += sp
+            same_w += sw
+            max_dsde = max(max_dsde, d)
+        cpu_value = len(done) / (t_cpu * total_macs / w_cpu) * 3600.0
         cpu = {"value": round(cpu_value, 2), "unit": "light curves/hour", "cores": cores,
                "kind": "port",
-               "sample": "%d stratified curves of the batch (%.1f s of FP64 oracle search "
-                         "on %d threads), extrapolated by sum N' x periods x widths"
-                         % (used, t_cpu, cores)}
-        log("cpu oracle: %.2f curves/h over %d curves" % (cpu_value, used))
+               "sample": "%d curves of the set spread over its work distribution "
+                         "(N %d-%d; %.1f s of FP64 oracle search on %d threads), "
+                         "extrapolated by window MACs" % (
+                             len(used), min(plans[i].n for i in used),
+                             max(plans[i].n for i in used), t_cpu, cores)}
+        extra.update({"parity_vs_cpu_same_best_period": "%d/%d" % (same_p, len(used)),
+                      "parity_vs_cpu_same_best_width": "%d/%d" % (same_w, len(used)),
+                      "parity_vs_cpu_max_rel_sde_err": float(max_dsde)})
+        log("cpu oracle: %.2f curves/h over %d curves, parity %d/%d %d/%d, max dSDE %.2e" % (
+            cpu_value, len(used), same_p, len(used), same_w, len(used), max_dsde))
 
     out = {
         "metric": "light curves/hour, batched FP32-comp vs plain FP64 on 1 GPU; same peak, "
@@ -213,11 +282,11 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64",
         "data": "synthetic (seeded light curves with injected limb-darkened transits)",
-        "config": {"workload": "batched TLS search, %d curves, N log-U(2000, 70000), "
-                               "cadence 2/30 min, own period grid per curve, "
-                               "T0_fit_margin 0.125" % args.curves,
-                   "curves": args.curves, "total_points": int(bp.pt_off[-1]),
-                   "total_periods": int(npd), "mode": "fp32c", "parallelism": "1 GPU"},
+        "config": {"workload": "batched TLS search, %d-curve set in batches of %d, "
+                               "N log-U(2000, 70000), cadence 2/30 min, own period grid "
+                               "per curve, T0_fit_margin 0.125" % (args.curves, args.batch),
+                   "curves": args.curves, "batch": args.batch,
+                   "curves_searched": n_done, "mode": "fp32c", "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,
