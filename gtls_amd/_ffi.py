@@ -43,6 +43,7 @@ class GtlsBatch(ct.Structure):
         ("skip_point", I64), ("depth_min", F64), ("negative_time", I32),
         ("total_points", I64), ("total_periods", I64), ("total_widths", I64),
         ("total_tmpl", I64), ("max_points", I32), ("max_width", I32),
+        ("max_period_widths", I32),
     ]
 
 

@@ -66,6 +66,8 @@ class BatchPlan:
         self.tmpl = np.concatenate([p.table.S for p in ps])
         self.max_points = int(max(p.n for p in ps))
         self.max_width = int(self.widths.max())
+        ok = self.k_lo <= self.k_hi
+        self.max_period_widths = int(max(1, np.max((self.k_hi - self.k_lo + 1)[ok], initial=1)))
         self.negative_time = int(np.min(self.t) < 0)
 
     @property
@@ -134,6 +136,7 @@ class DeviceBatch:
         b.total_tmpl = int(len(bp.tmpl))
         b.max_points = bp.max_points
         b.max_width = bp.max_width
+        b.max_period_widths = bp.max_period_widths
         return b
 
     def spectra(self, chi2, oversampling_factor=C.OVERSAMPLING_FACTOR, sync=True):

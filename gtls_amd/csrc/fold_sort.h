@@ -9,12 +9,13 @@
 // phases gives (src/gputls/core.py:775-777). One most-significant-digit radix pass
 // distributes the samples over B phase buckets (bucket = floor(phase * B), monotone in
 // the phase): an LDS histogram, an LDS exclusive scan, and a scatter whose slot inside a
-// bucket comes from an LDS atomic. The exact order inside each bucket is then fixed by an
-// insertion sort on the full (phase, index) key, so the scatter order does not matter.
-// With B ~ N/4 a bucket holds a handful of samples, so a period costs one read of t, one
-// scatter and a few compares per sample instead of the 6-8 passes of a 64-bit LSD sort.
-// Phases that crowd into one bucket (periods close to a multiple of the cadence) only
-// make that bucket's insertion sort longer; the result is exact in every case.
+// bucket comes from an LDS atomic. The exact order inside each bucket is then fixed on the
+// full (phase, index) key, so the scatter order does not matter. With B ~ N/4 a bucket
+// holds a handful of samples (a register sorting network per thread); phases that crowd
+// into one bucket (periods close to a multiple of the cadence: tens to hundreds of
+// samples) are sorted by one wavefront with a bitonic network in registers, larger ones
+// by the workgroup in LDS, and the largest (e.g. repeated times) by LDS-sorted chunks and
+// rank-based merges. The cost of a bucket is bounded by m log^2 m in every case.
 #pragma once
 #include "wg_primitives.h"
 
@@ -109,20 +110,188 @@ __device__ __forceinline__ void sort_bucket_regs(double* __restrict__ key, int32
     }
 }
 
+constexpr int kWaveSortMax = 256;    // buckets up to this size: one wavefront, registers
+constexpr int kBlockSortMax = 2048;  // up to this size: the workgroup, in LDS
+
+// Bitonic sort of cnt <= 64 E keys by one wavefront: element i = lane E + e sits in
+// register e of its lane; partners i ^ j with j < E are in the same lane, the others one
+// shuffle away. Padding keys (+inf, INT_MAX) sort last.
+template <int E>
+__device__ __forceinline__ void wave_bitonic(double* __restrict__ key, int32_t* __restrict__ id,
+                                             int64_t lo, int cnt) {
+    const int lane = lane_id();
+    double k[E];
+    int32_t x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = lane * E + e;
+        const bool in = i < cnt;
+        k[e] = in ? key[lo + (in ? i : 0)] : __builtin_inf();
+        x[e] = in ? id[lo + (in ? i : 0)] : 0x7fffffff;
+    }
+    constexpr int M = 64 * E;
+#pragma unroll
+    for (int kk = 2; kk <= M; kk <<= 1) {
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            if (j >= E) {
+                const int lj = j / E;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = lane * E + e;
+                    const double pk = __shfl_xor(k[e], lj, kWave);
+                    const int32_t px = __shfl_xor(x[e], lj, kWave);
+                    const bool want_min = ((i & j) == 0) == ((i & kk) == 0);
+                    const bool pg = key_greater(pk, px, k[e], x[e]);   // partner > mine
+                    const bool take = want_min ? !pg : pg;              // partner's is the pick
+                    if (take) {
+                        k[e] = pk;
+                        x[e] = px;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if ((e & j) == 0) {
+                        const int i = lane * E + e;
+                        const bool asc = (i & kk) == 0;
+                        if (key_greater(k[e], x[e], k[e | j], x[e | j]) == asc) {
+                            const double tk = k[e];
+                            const int32_t tx = x[e];
+                            k[e] = k[e | j];
+                            x[e] = x[e | j];
+                            k[e | j] = tk;
+                            x[e | j] = tx;
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = lane * E + e;
+        if (i < cnt) {
+            key[lo + i] = k[e];
+            id[lo + i] = x[e];
+        }
+    }
+}
+
+// Bitonic sort of cnt <= kBlockSortMax keys by the whole workgroup in LDS (sk: doubles,
+// si: ints, kBlockSortMax each). All threads call it with the same arguments.
+__device__ void block_bitonic(double* __restrict__ key, int32_t* __restrict__ id, int64_t lo,
+                              int cnt, double* sk, int32_t* si) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int M = 64;
+    while (M < cnt) M <<= 1;
+    for (int i = tid; i < M; i += nt) {
+        const bool in = i < cnt;
+        sk[i] = in ? key[lo + (in ? i : 0)] : __builtin_inf();
+        si[i] = in ? id[lo + (in ? i : 0)] : 0x7fffffff;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= M; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < M; i += nt) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const bool asc = (i & kk) == 0;
+                    if (key_greater(sk[i], si[i], sk[p], si[p]) == asc) {
+                        const double tk = sk[i];
+                        const int32_t tx = si[i];
+                        sk[i] = sk[p];
+                        si[i] = si[p];
+                        sk[p] = tk;
+                        si[p] = tx;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < cnt; i += nt) {
+        key[lo + i] = sk[i];
+        id[lo + i] = si[i];
+    }
+    __syncthreads();
+}
+
+// Number of keys in the sorted run (rk, ri)[0, len) less than (k, x).
+__device__ __forceinline__ int rank_in(const double* rk, const int32_t* ri, int len, double k,
+                                       int32_t x) {
+    int a = 0, b = len;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (key_greater(k, x, rk[m], ri[m])) a = m + 1; else b = m;
+    }
+    return a;
+}
+
+// Sort of any bucket size by the workgroup: kBlockSortMax-key chunks in LDS, then merge
+// passes between (key, id) and the scratch (tk, ti) in which every key's place comes from
+// its rank in the other run (keys are unique: the index breaks phase ties).
+__device__ void block_merge_sort(double* __restrict__ key, int32_t* __restrict__ id,
+                                 double* __restrict__ tk, int32_t* __restrict__ ti, int64_t lo,
+                                 int cnt, double* sk, int32_t* si) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int c0 = 0; c0 < cnt; c0 += kBlockSortMax)
+        block_bitonic(key, id, lo + c0, min(kBlockSortMax, cnt - c0), sk, si);
+    double* sk_ = key + lo;
+    int32_t* si_ = id + lo;
+    double* dk = tk + lo;
+    int32_t* di = ti + lo;
+    for (int w = kBlockSortMax; w < cnt; w <<= 1) {
+        for (int i = tid; i < cnt; i += nt) {
+            const int a0 = (i / (2 * w)) * (2 * w);
+            const int b0 = min(a0 + w, cnt), b1 = min(a0 + 2 * w, cnt);
+            const double k = sk_[i];
+            const int32_t x = si_[i];
+            int pos;
+            if (i < b0) pos = a0 + (i - a0) + rank_in(sk_ + b0, si_ + b0, b1 - b0, k, x);
+            else pos = a0 + (i - b0) + rank_in(sk_ + a0, si_ + a0, b0 - a0, k, x);
+            dk[pos] = k;
+            di[pos] = x;
+        }
+        __syncthreads();
+        double* t0 = sk_; sk_ = dk; dk = t0;
+        int32_t* t1 = si_; si_ = di; di = t1;
+    }
+    if (sk_ != key + lo) {
+        for (int i = tid; i < cnt; i += nt) {
+            key[lo + i] = sk_[i];
+            id[lo + i] = si_[i];
+        }
+        __syncthreads();
+    }
+}
+
 // Sorts the n samples of t by (phase, index) for period P. Writes the permutation to
-// perm[0..n) and the sorted phases to key_out[0..n); key_in is n doubles of scratch.
-// hist is LDS of kMaxBuckets ints, red is LDS of blockDim/64 ints. All threads call it.
-__device__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, bool neg,
-                             double* __restrict__ key_in, double* __restrict__ key_out,
-                             int32_t* __restrict__ perm, int* hist, int* red) {
+// perm[0..n) and the sorted phases to key_out[0..n). Scratch: tmp_key, tmp_id (n each) and
+// lists (kMaxBuckets int64). LDS: hist (kMaxBuckets ints, 16-byte aligned), red
+// (blockDim/64 ints), counts (2 ints). All threads call it.
+__device__ __forceinline__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, bool neg,
+                             double* __restrict__ key_out, int32_t* __restrict__ perm,
+                             double* __restrict__ tmp_key, int32_t* __restrict__ tmp_id,
+                             int64_t* __restrict__ lists, int* hist, int* red, int* counts) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int B = buckets_for(n);
     for (int b = tid; b < B; b += nt) hist[b] = 0;
+    if (tid < 2) counts[tid] = 0;
     __syncthreads();
-    for (int64_t j = tid; j < n; j += nt) {
-        const double ph = fold_phase(t[j], P);
-        key_in[j] = ph;
-        atomicAdd(&hist[phase_bucket(ph, B, neg)], 1);
+    constexpr int U = 4;   // samples in flight per thread
+    for (int64_t base = 0; base < n; base += (int64_t)U * nt) {
+        double tv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = base + (int64_t)u * nt + tid;
+            tv[u] = j < n ? t[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = base + (int64_t)u * nt + tid;
+            if (j < n) atomicAdd(&hist[phase_bucket(fold_phase(tv[u], P), B, neg)], 1);
+        }
     }
     __syncthreads();
     // exclusive scan of the histogram: thread-contiguous chunks, then a workgroup scan
@@ -140,16 +309,29 @@ __device__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, 
         }
     }
     __syncthreads();
-    for (int64_t j = tid; j < n; j += nt) {
-        const double ph = key_in[j];
-        const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1);
-        perm[pos] = (int32_t)j;
-        key_out[pos] = ph;
+    // scatter, folding again (t is re-read from cache rather than phases stored)
+    for (int64_t base = 0; base < n; base += (int64_t)U * nt) {
+        double tv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = base + (int64_t)u * nt + tid;
+            tv[u] = j < n ? t[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = base + (int64_t)u * nt + tid;
+            if (j < n) {
+                const double ph = fold_phase(tv[u], P);
+                const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1);
+                perm[pos] = (int32_t)j;
+                key_out[pos] = ph;
+            }
+        }
     }
     __syncthreads();
     // hist[b] now holds the end of bucket b; order each bucket by (phase, index): up to 16
-    // keys in registers through a sorting network (independent loads, no dependent
-    // memory round trips), larger buckets by insertion in place
+    // keys in registers through a sorting network, larger buckets are listed (from the
+    // front up to kWaveSortMax keys, from the back beyond) for the passes below
     for (int b = tid; b < B; b += nt) {
         const int lo = b ? hist[b - 1] : 0;
         const int hi = hist[b];
@@ -157,24 +339,41 @@ __device__ void wg_fold_sort(const double* __restrict__ t, int64_t n, double P, 
         if (cnt <= 1) continue;
         if (cnt <= 8) {
             sort_bucket_regs<8>(key_out, perm, lo, cnt);
-            continue;
-        }
-        if (cnt <= 16) {
+        } else if (cnt <= 16) {
             sort_bucket_regs<16>(key_out, perm, lo, cnt);
-            continue;
+        } else if (cnt <= kWaveSortMax) {
+            const int e = atomicAdd(&counts[0], 1);
+            lists[e] = ((int64_t)lo << 32) | (int64_t)cnt;
+        } else {
+            const int e = atomicAdd(&counts[1], 1);
+            lists[kMaxBuckets - 1 - e] = ((int64_t)lo << 32) | (int64_t)cnt;
         }
-        for (int i = lo + 1; i < hi; ++i) {
-            const double k = key_out[i];
-            const int32_t id = perm[i];
-            int j = i - 1;
-            while (j >= lo && key_greater(key_out[j], perm[j], k, id)) {
-                key_out[j + 1] = key_out[j];
-                perm[j + 1] = perm[j];
-                --j;
-            }
-            key_out[j + 1] = k;
-            perm[j + 1] = id;
+    }
+    __syncthreads();
+    const int nmid = counts[0], nbig = counts[1];
+    // 17 .. kWaveSortMax keys: one wavefront each
+    {
+        const int wave = uniform((int32_t)wave_id());
+        const int nwaves = nt / kWave;
+        for (int e = wave; e < nmid; e += nwaves) {
+            const int64_t ent = uniform(lists[e]);
+            const int64_t lo = ent >> 32;
+            const int cnt = (int)(ent & 0xffffffffLL);
+            if (cnt <= 64) wave_bitonic<1>(key_out, perm, lo, cnt);
+            else if (cnt <= 128) wave_bitonic<2>(key_out, perm, lo, cnt);
+            else wave_bitonic<4>(key_out, perm, lo, cnt);
         }
+    }
+    __syncthreads();
+    // larger buckets: the workgroup, in LDS (the histogram is no longer needed)
+    double* sk = (double*)hist;
+    int32_t* si = (int32_t*)(sk + kBlockSortMax);
+    for (int e = 0; e < nbig; ++e) {
+        const int64_t ent = uniform(lists[kMaxBuckets - 1 - e]);
+        const int64_t lo = ent >> 32;
+        const int cnt = (int)(ent & 0xffffffffLL);
+        if (cnt <= kBlockSortMax) block_bitonic(key_out, perm, lo, cnt, sk, si);
+        else block_merge_sort(key_out, perm, tmp_key, tmp_id, lo, cnt, sk, si);
     }
     __syncthreads();
 }

@@ -16,10 +16,11 @@ struct gtls_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int n_cu = 0;
-    int slots_fp64 = 0;
-    int slots_fp32 = 0;
+    int prep_fp64 = 0, prep_fp32 = 0;   // resident workgroups of prep_kernel
+    int win_fp64 = 0, win_fp32 = 0;     // resident workgroups of windows_kernel
     int slots_sort = 0;
     int use_mfma = 1;
+    int64_t chunk_bytes = 4LL << 30;    // budget of one chunk's gathered samples
     std::string err;
 };
 
@@ -54,52 +55,72 @@ int hip_fail(gtls_ctx* ctx, hipError_t e, const char* where) {
 }
 
 struct SearchLayout {
-    int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, ss = 0, wconst = 0, flat = 0, queue = 0, slots = 0;
-    int64_t slot_bytes = 0, o_key_in = 0, o_key_out = 0, o_perm = 0, o_vw = 0, o_D = 0;
-    int64_t n_slots = 0, total = 0;
+    int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, ss = 0, wconst = 0, flat = 0, counters = 0;
+    int64_t slots = 0, slot_bytes = 0, o_key = 0, o_perm = 0, o_tmp_key = 0, o_tmp_id = 0;
+    int64_t o_lists = 0, n_slots = 0;
+    int64_t xbuf = 0, dbuf = 0, meta = 0, part = 0;
+    int64_t xstride = 0, chunk = 0, n_chunks = 0, kmax = 0, n_win = 0;
+    int64_t total = 0;
 };
 
-SearchLayout plan_search(const gtls_ctx* ctx, int mode, int64_t total_points,
-                         int64_t total_tmpl, int64_t total_widths, int32_t n_curves,
-                         int64_t max_points, int64_t max_width) {
+// Workspace: per-launch prepared copies, per-workgroup sort scratch, and the buffers of one
+// chunk of periods (gathered samples, running sums, per-period meta, per-width minima).
+SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     const int64_t real = (mode == GTLS_MODE_FP64) ? 8 : 4;
+    const int64_t max_points = b->max_points, max_width = b->max_width > 0 ? b->max_width : 1;
     SearchLayout L;
     int64_t off = 0;
     L.uw0 = off;
-    off = align_up(off + 2 * real * total_points);
+    off = align_up(off + 2 * real * b->total_points);
     // rows with kTmplPad zeros on both sides, then a zeroed tail for the over-reads of the
     // matrix-core path (up to one block, i.e. max_width samples, past a row)
-    const int64_t tmpl_padded = total_tmpl + 2 * kTmplPad * total_widths + kTmplTail;
+    const int64_t tmpl_padded = b->total_tmpl + 2 * kTmplPad * b->total_widths + kTmplTail;
     L.tmpl2 = off;
     off = align_up(off + 2 * real * tmpl_padded);
     L.tmpl1 = off;
     off = align_up(off + real * tmpl_padded);
     L.ss = off;
-    off = align_up(off + 8 * (total_widths > 0 ? total_widths : 1));
+    off = align_up(off + 8 * (b->total_widths > 0 ? b->total_widths : 1));
     L.wconst = off;
-    off = align_up(off + 8 * (int64_t)n_curves);
+    off = align_up(off + 8 * (int64_t)b->n_curves);
     L.flat = off;
-    off = align_up(off + 8 * (int64_t)n_curves);
-    L.queue = off;
-    off = align_up(off + 8);
-    // gathered samples: N + widest width of the wrap, plus up to another max_width read
-    // (and discarded) by the last lane of a register-blocked window
-    const int64_t np_ = max_points + 2 * (max_width > 0 ? max_width : 1) + 64;
+    off = align_up(off + 8 * (int64_t)b->n_curves);
+    // sort scratch of one prep workgroup
     int64_t s = 0;
-    L.o_key_in = s;
-    s = align_up(s + 8 * max_points);
-    L.o_key_out = s;
+    L.o_key = s;
     s = align_up(s + 8 * max_points);
     L.o_perm = s;
     s = align_up(s + 4 * max_points);
-    L.o_vw = s;
-    s = align_up(s + 2 * real * np_);
-    L.o_D = s;
-    s = align_up(s + 8 * np_);
+    L.o_tmp_key = s;
+    s = align_up(s + 8 * max_points);
+    L.o_tmp_id = s;
+    s = align_up(s + 4 * max_points);
+    L.o_lists = s;
+    s = align_up(s + 8 * (int64_t)kMaxBuckets);
     L.slot_bytes = s;
-    L.n_slots = (mode == GTLS_MODE_FP64) ? ctx->slots_fp64 : ctx->slots_fp32;
+    L.n_slots = (mode == GTLS_MODE_FP64) ? ctx->prep_fp64 : ctx->prep_fp32;
+    L.n_win = (mode == GTLS_MODE_FP64) ? ctx->win_fp64 : ctx->win_fp32;
     L.slots = off;
-    off += L.slot_bytes * L.n_slots;
+    off = align_up(off + L.slot_bytes * L.n_slots);
+    // per period of a chunk: gathered samples N + widest width of the wrap, plus up to
+    // another max_width read (and discarded) past them by the window loops
+    L.xstride = max_points + 2 * max_width + 64;
+    L.kmax = b->max_period_widths;
+    const int64_t per_period =
+        L.xstride * (2 * real + 8) + (int64_t)sizeof(PeriodMeta) + L.kmax * (int64_t)sizeof(ArgMin);
+    const int64_t periods = b->total_periods > 0 ? b->total_periods : 1;
+    L.chunk = std::max<int64_t>(1, std::min<int64_t>(periods, ctx->chunk_bytes / per_period));
+    L.n_chunks = (periods + L.chunk - 1) / L.chunk;
+    L.counters = off;
+    off = align_up(off + 16 * 8 * L.n_chunks);
+    L.xbuf = off;
+    off = align_up(off + 2 * real * L.xstride * L.chunk);
+    L.dbuf = off;
+    off = align_up(off + 8 * L.xstride * L.chunk);
+    L.meta = off;
+    off = align_up(off + (int64_t)sizeof(PeriodMeta) * L.chunk);
+    L.part = off;
+    off = align_up(off + (int64_t)sizeof(ArgMin) * L.kmax * L.chunk);
     L.total = align_up(off);
     return L;
 }
@@ -126,7 +147,17 @@ bool batch_ok(const gtls_batch* b, std::string& why) {
         return false;
     }
     if (b->skip_point <= 0) { why = "skip_point must be positive"; return false; }
+    if (b->max_period_widths <= 0 || b->max_period_widths > (1 << 20)) {
+        why = "max_period_widths out of range";
+        return false;
+    }
     return true;
+}
+
+__global__ void zero_u64s_kernel(unsigned long long* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ULL;
 }
 
 template <class Real>
@@ -140,7 +171,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     double* ss = (double*)(ws + L.ss);
     double* wconst = (double*)(ws + L.wconst);
     double* flat = (double*)(ws + L.flat);
-    unsigned long long* queue = (unsigned long long*)(ws + L.queue);
+    unsigned long long* counters = (unsigned long long*)(ws + L.counters);
 
     const int tpb = 256;
     hipLaunchKernelGGL(prep_points_kernel<Real>, dim3((unsigned)((b->total_points + tpb - 1) / tpb)),
@@ -152,7 +183,9 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
                        st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, ss);
     hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
                        b->y, b->dy, flat, wconst);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, st, queue);
+    const int64_t ncount = 16 * L.n_chunks;
+    hipLaunchKernelGGL(zero_u64s_kernel, dim3((unsigned)std::min<int64_t>(1024, (ncount + tpb - 1) / tpb)),
+                       dim3(tpb), 0, st, counters, ncount);
 
     SearchParams<Real> p;
     p.n_curves = b->n_curves;
@@ -175,24 +208,36 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.skip_point = b->skip_point;
     p.depth_min = b->depth_min;
     p.negative_time = b->negative_time;
-    p.total_periods = b->total_periods;
     p.use_mfma = ctx->use_mfma;
     p.max_points = b->max_points;
     p.max_width = b->max_width > 0 ? b->max_width : 1;
-    p.queue = queue;
+    p.kmax = (int32_t)L.kmax;
     p.slots = ws + L.slots;
     p.slot_bytes = L.slot_bytes;
-    p.o_key_in = L.o_key_in;
-    p.o_key_out = L.o_key_out;
+    p.o_key = L.o_key;
     p.o_perm = L.o_perm;
-    p.o_vw = L.o_vw;
-    p.o_D = L.o_D;
+    p.o_tmp_key = L.o_tmp_key;
+    p.o_tmp_id = L.o_tmp_id;
+    p.o_lists = L.o_lists;
+    p.xbuf = (R2*)(ws + L.xbuf);
+    p.dbuf = (double*)(ws + L.dbuf);
+    p.xstride = L.xstride;
+    p.meta = (PeriodMeta*)(ws + L.meta);
+    p.part = (ArgMin*)(ws + L.part);
     p.chi2 = chi2;
     p.best_k = best_k;
     p.best_s = best_s;
-    if (b->total_periods > 0) {
-        const int64_t grid = std::min<int64_t>(L.n_slots, b->total_periods);
-        hipLaunchKernelGGL(tls_search_kernel<Real>, dim3((unsigned)grid), dim3(kBlock), 0, st, p);
+    for (int64_t ci = 0; ci < L.n_chunks && b->total_periods > 0; ++ci) {
+        p.p0 = ci * L.chunk;
+        p.chunk = std::min<int64_t>(L.chunk, b->total_periods - p.p0);
+        p.counters = counters + 16 * ci;
+        const int64_t g1 = std::min<int64_t>(L.n_slots, p.chunk);
+        hipLaunchKernelGGL(prep_kernel<Real>, dim3((unsigned)g1), dim3(kBlock), 0, st, p);
+        const int64_t g2 = std::min<int64_t>(L.n_win, (p.chunk * L.kmax + 3) / 4);
+        hipLaunchKernelGGL(windows_kernel<Real>, dim3((unsigned)std::max<int64_t>(1, g2)),
+                           dim3(kBlock), 0, st, p);
+        hipLaunchKernelGGL(reduce_kernel<Real>, dim3((unsigned)((p.chunk + tpb - 1) / tpb)),
+                           dim3(tpb), 0, st, p);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "search launch");
@@ -228,8 +273,14 @@ int gtls_create(int device, void* stream, gtls_ctx** out) {
     ctx->stream = (hipStream_t)stream;
     ctx->n_cu = prop.multiProcessorCount;
     if (const char* e = std::getenv("GTLS_MFMA")) ctx->use_mfma = std::atoi(e) != 0;
-    ctx->slots_fp64 = occupancy_slots(ctx, (const void*)tls_search_kernel<double>);
-    ctx->slots_fp32 = occupancy_slots(ctx, (const void*)tls_search_kernel<float>);
+    if (const char* e = std::getenv("GTLS_CHUNK_MB")) {
+        const long long mb = std::atoll(e);
+        if (mb > 0) ctx->chunk_bytes = mb << 20;
+    }
+    ctx->prep_fp64 = occupancy_slots(ctx, (const void*)prep_kernel<double>);
+    ctx->prep_fp32 = occupancy_slots(ctx, (const void*)prep_kernel<float>);
+    ctx->win_fp64 = occupancy_slots(ctx, (const void*)windows_kernel<double>);
+    ctx->win_fp32 = occupancy_slots(ctx, (const void*)windows_kernel<float>);
     ctx->slots_sort = occupancy_slots(ctx, (const void*)fold_sort_kernel);
     *out = ctx;
     return GTLS_OK;
@@ -245,15 +296,14 @@ int gtls_set_stream(gtls_ctx* ctx, void* stream) {
 
 const char* gtls_last_error(const gtls_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-int32_t gtls_search_slots(const gtls_ctx* ctx) { return ctx ? ctx->slots_fp32 : 0; }
+int32_t gtls_search_slots(const gtls_ctx* ctx) { return ctx ? ctx->prep_fp32 : 0; }
 
 int64_t gtls_search_workspace_bytes(gtls_ctx* ctx, int32_t mode, const gtls_batch* b) {
     if (!ctx) return GTLS_E_SHAPE;
     if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C) return fail(ctx, GTLS_E_MODE, "unknown mode");
     std::string why;
     if (!batch_ok(b, why)) return fail(ctx, GTLS_E_SHAPE, why);
-    return plan_search(ctx, mode, b->total_points, b->total_tmpl, b->total_widths,
-                       b->n_curves, b->max_points, b->max_width).total;
+    return plan_search(ctx, mode, b).total;
 }
 
 int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* b, void* workspace,
@@ -265,9 +315,7 @@ int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* b, void* 
     if (!batch_ok(b, why)) return fail(ctx, GTLS_E_SHAPE, why);
     if (!workspace || !chi2 || !best_k || !best_s) return fail(ctx, GTLS_E_SHAPE, "NULL output or workspace");
     DeviceGuard guard(ctx->device);
-    const SearchLayout L = plan_search(ctx, mode, b->total_points, b->total_tmpl,
-                                       b->total_widths, b->n_curves, b->max_points,
-                                       b->max_width);
+    const SearchLayout L = plan_search(ctx, mode, b);
     if (workspace_bytes < L.total)
         return fail(ctx, GTLS_E_WORKSPACE, "workspace too small: need " + std::to_string(L.total) +
                                                " bytes, got " + std::to_string(workspace_bytes));
@@ -295,7 +343,17 @@ int64_t gtls_search_workspace_bytes_1(gtls_ctx* ctx, int32_t mode, int64_t n,
     if (n <= 0 || n_periods < 0 || n_widths <= 0 || total_tmpl < 0 || max_width < 0 ||
         n > (1LL << 30))
         return fail(ctx, GTLS_E_SHAPE, "sizes out of range");
-    return kSingleHead + plan_search(ctx, mode, n, total_tmpl, n_widths, 1, n, max_width).total;
+    gtls_batch b;
+    std::memset(&b, 0, sizeof(b));
+    b.n_curves = 1;
+    b.total_points = n;
+    b.total_periods = n_periods;
+    b.total_widths = n_widths;
+    b.total_tmpl = total_tmpl;
+    b.max_points = (int32_t)n;
+    b.max_width = max_width;
+    b.max_period_widths = n_widths;
+    return kSingleHead + plan_search(ctx, mode, &b).total;
 }
 
 int gtls_search(gtls_ctx* ctx, int32_t mode, const double* t, const double* y,
@@ -342,6 +400,7 @@ int gtls_search(gtls_ctx* ctx, int32_t mode, const double* t, const double* y,
     b.total_tmpl = total_tmpl;
     b.max_points = (int32_t)n;
     b.max_width = max_width;
+    b.max_period_widths = n_widths;
     return gtls_search_batched(ctx, mode, &b, (char*)workspace + kSingleHead,
                                workspace_bytes - kSingleHead, chi2, best_k, best_s);
 }
@@ -349,7 +408,7 @@ int gtls_search(gtls_ctx* ctx, int32_t mode, const double* t, const double* y,
 int64_t gtls_fold_sort_workspace_bytes(gtls_ctx* ctx, int64_t n) {
     if (!ctx) return GTLS_E_SHAPE;
     if (n <= 0 || n > (1LL << 30)) return fail(ctx, GTLS_E_SHAPE, "n out of range");
-    return align_up(16 * n) * ctx->slots_sort;
+    return fold_sort_slot_bytes(n) * ctx->slots_sort;
 }
 
 int gtls_fold_sort(gtls_ctx* ctx, const double* t, int64_t n, const double* periods,
@@ -365,8 +424,7 @@ int gtls_fold_sort(gtls_ctx* ctx, const double* t, int64_t n, const double* peri
     DeviceGuard guard(ctx->device);
     const int64_t grid = std::min<int64_t>(ctx->slots_sort, n_periods);
     hipLaunchKernelGGL(fold_sort_kernel, dim3((unsigned)grid), dim3(kBlock), 0, ctx->stream, t,
-                       n, periods, n_periods, negative_time, (char*)workspace, align_up(16 * n),
-                       perm);
+                       n, periods, n_periods, negative_time, (char*)workspace, perm);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "fold_sort launch");
     return GTLS_OK;

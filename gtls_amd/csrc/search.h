@@ -30,6 +30,13 @@ template <class Real> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
 
+// Per trial period of a chunk: written by prep_kernel, read by the two later stages.
+struct PeriodMeta {
+    int32_t c;        // curve
+    int32_t status;   // 0 searched, -1 no widths / no points (NaN), -2 inputs overrun a slot
+    int64_t np;       // gathered samples: N + widest searched width
+};
+
 template <class Real>
 struct SearchParams {
     using R2 = typename Pair<Real>::type;
@@ -53,14 +60,25 @@ struct SearchParams {
     int64_t skip_point;
     double depth_min;
     int32_t negative_time;
-    int64_t total_periods;
     int32_t use_mfma;     // float32: window sums on the matrix cores
-    int32_t max_points;   // slot capacity: every curve must have n <= max_points
-    int32_t max_width;    // and every searched width d <= max_width
-    unsigned long long* queue;
+    int32_t max_points;   // every curve must have n <= max_points
+    int32_t max_width;    // every searched width must be <= max_width
+    int32_t kmax;         // items per period in the window stage (>= widths per period)
+    // the chunk: periods [p0, p0 + chunk) of the batch
+    int64_t p0;
+    int64_t chunk;
+    // prep_kernel's per-workgroup sort scratch
     char* slots;
     int64_t slot_bytes;
-    int64_t o_key_in, o_key_out, o_perm, o_vw, o_D;
+    int64_t o_key, o_perm, o_tmp_key, o_tmp_id, o_lists;
+    // per chunk period i: gathered samples xbuf[i * xstride ...] (u, or (u w, w) pairs)
+    // and their float64 running sum dbuf[i * xstride ...]
+    R2* xbuf;
+    double* dbuf;
+    int64_t xstride;
+    PeriodMeta* meta;
+    ArgMin* part;                  // per (chunk period, width slot): the width's minimum
+    unsigned long long* counters;  // [0] prep queue, [1 .. 8] window queues (one per XCD)
     double* chi2;
     int32_t* best_k;
     int32_t* best_s;
@@ -82,71 +100,74 @@ __device__ __forceinline__ void consider(ArgMin& b, double v, int32_t k, int32_t
     if (argmin_less(c, b)) b = c;
 }
 
+// The XCD (accelerator complex die) this wavefront runs on, 0..7.
+__device__ __forceinline__ int xcd_id() {
+    int v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(v));
+    return v & 7;
+}
+
+// Stage 1 + 2 of each period of the chunk, one workgroup per period (persistent):
+//   1. fold + stable phase sort (fold_sort.h)                  -> perm
+//   2. gather in phase order with the wrap-around extension    -> X, running sum D
 template <class Real>
-__global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p) {
+__global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
     using R2 = typename Pair<Real>::type;
-    __shared__ int hist[kMaxBuckets];
+    __shared__ __attribute__((aligned(16))) int hist[kMaxBuckets];
     __shared__ int red_i[kBlock / kWave];
+    __shared__ int s_counts[2];
     __shared__ double red_d[kBlock / kWave];
-    __shared__ ArgMin red_a[kBlock / kWave];
     __shared__ unsigned long long s_item;
-    __shared__ float rings[2][kBlock / kWave][kRingRows * kRingStride];
 
     const int tid = threadIdx.x;
     char* slot = p.slots + (int64_t)blockIdx.x * p.slot_bytes;
-    double* key_in = (double*)(slot + p.o_key_in);
-    double* key_out = (double*)(slot + p.o_key_out);
+    double* key = (double*)(slot + p.o_key);
     int32_t* perm = (int32_t*)(slot + p.o_perm);
-    R2* vw = (R2*)(slot + p.o_vw);
-    double* D = (double*)(slot + p.o_D);
+    double* tmp_key = (double*)(slot + p.o_tmp_key);
+    int32_t* tmp_id = (int32_t*)(slot + p.o_tmp_id);
+    int64_t* lists = (int64_t*)(slot + p.o_lists);
 
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(p.queue, 1ULL);
+        if (tid == 0) s_item = atomicAdd(&p.counters[0], 1ULL);
         __syncthreads();
-        const int64_t g = uniform((int64_t)s_item);
+        const int64_t i = uniform((int64_t)s_item);
         __syncthreads();
-        if (g >= p.total_periods) break;
+        if (i >= p.chunk) break;
+        const int64_t g = p.p0 + i;
 
-        // Every value below derives from g and is wave-uniform; the item either runs all
-        // three stages or only writes its status, and both paths meet at the barrier
-        // that ends the iteration (no early continue out of a loop with barriers).
+        // Every value below derives from g and is wave-uniform; the item either runs both
+        // stages or only writes its status, and both paths meet at the barrier that ends
+        // the iteration (no early continue out of a loop with barriers).
         const int c = find_owner(p.per_off, p.n_curves, g);
         const int64_t pt0 = p.pt_off[c];
         const int64_t n = p.pt_off[c + 1] - pt0;
         const int64_t w0 = p.wid_off[c];
         const int32_t klo = p.k_lo[g], khi = p.k_hi[g];
-        // status: 0 run, -1 no widths (NaN), -2 inputs that would overrun the slot
         int32_t status = (klo > khi || n <= 0) ? -1 : 0;
         int32_t dmax = 1;
         if (status == 0) {
             const int64_t nw = p.wid_off[c + 1] - w0;
-            if (klo < 0 || khi >= nw || n > p.max_points) {
+            if (klo < 0 || khi >= nw || n > p.max_points || khi - klo + 1 > p.kmax) {
                 status = -2;
             } else {
                 for (int32_t k = klo; k <= khi; ++k) dmax = max(dmax, p.widths[w0 + k]);
                 if (dmax > p.max_width) status = -2;
             }
         }
-        if (status != 0) {
-            if (tid == 0) {
-                p.chi2[g] = __builtin_nan("");
-                p.best_k[g] = status;
-                p.best_s[g] = status;
-            }
-        } else {
-        const double P = p.periods[g];
-
-        // 1. fold + sort
-        wg_fold_sort(p.t + pt0, n, P, p.negative_time != 0, key_in, key_out, perm, hist,
-                     red_i);
-
-        // 2. gather in phase order (wrapping) and the float64 running sum of u. With a
-        //    common weight only u is kept (X = u), otherwise the pairs (u w, w).
-        const double wc = p.wconst[c];
-        const bool constw = wc == wc;
-        Real* ub = (Real*)vw;
         const int64_t np_ = n + dmax;
-        {
+        if (tid == 0) p.meta[i] = PeriodMeta{c, status, np_};
+        if (status == 0) {
+            const double P = p.periods[g];
+            wg_fold_sort(p.t + pt0, n, P, p.negative_time != 0, key, perm, tmp_key, tmp_id,
+                         lists, hist, red_i, s_counts);
+
+            // gather in phase order (wrapping) and the float64 running sum of u. With a
+            // common weight only u is kept (X = u), otherwise the pairs (u w, w).
+            const double wc = p.wconst[c];
+            const bool constw = wc == wc;
+            R2* vw = p.xbuf + i * p.xstride;
+            Real* ub = (Real*)vw;
+            double* D = p.dbuf + i * p.xstride;
             const int lane = lane_id(), wave = wave_id();
             const int64_t chunk = (int64_t)kBlock * kScanPerThread;
             double carry = 0.0;
@@ -156,20 +177,20 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
                 double run = 0.0;
 #pragma unroll
                 for (int e = 0; e < kScanPerThread; ++e) {
-                    const int64_t i = wbase + (int64_t)e * kWave + lane;
+                    const int64_t j = wbase + (int64_t)e * kWave + lane;
                     Real uu = 0, ww = 0;
-                    if (i < np_) {
-                        const int64_t j = i < n ? i : i % n;
-                        const R2 a = p.uw0[pt0 + perm[j]];
+                    if (j < np_) {
+                        const int64_t jj = j < n ? j : j % n;
+                        const R2 a = p.uw0[pt0 + perm[jj]];
                         uu = a.x;
                         ww = a.y;
                         if (constw) {
-                            ub[i] = uu;
+                            ub[j] = uu;
                         } else {
                             R2 o;
                             o.x = uu * ww;
                             o.y = ww;
-                            vw[i] = o;
+                            vw[j] = o;
                         }
                     }
                     const double s = wave_inclusive_scan((double)uu);
@@ -186,8 +207,8 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
                 __syncthreads();
 #pragma unroll
                 for (int e = 0; e < kScanPerThread; ++e) {
-                    const int64_t i = wbase + (int64_t)e * kWave + lane;
-                    if (i < np_) D[i] = u[e] + before;
+                    const int64_t j = wbase + (int64_t)e * kWave + lane;
+                    if (j < np_) D[j] = u[e] + before;
                 }
                 carry = total;
             }
@@ -199,131 +220,208 @@ __global__ __launch_bounds__(kBlock) void tls_search_kernel(SearchParams<Real> p
             }
         }
         __syncthreads();
-
-        // 3. windows: wavefront w takes widths klo + w, klo + w + nwaves, ...; every
-        //    per-width value is made wave-uniform so that template reads are scalar loads
-        //    and the segment predicates are scalar branches
-        const double chi2f = p.chi2_flat[c];
-        const double sentinel = (double)n;
-        ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
-        {
-            const int lane = lane_id();
-            const int wave = uniform((int32_t)wave_id());
-            const int nwaves = kBlock / kWave;
-            for (int32_t k = klo + wave; k <= khi; k += nwaves) {
-                const int64_t wg = w0 + k;                      // global width index
-                const int32_t d = uniform(p.widths[wg]);
-                if (d <= 0) {  // empty window: mean 0/0, every position carries the sentinel
-                    if (lane == 0) consider<Real>(best, sentinel, k, 0);
-                    continue;
-                }
-                const int32_t q = (int32_t)((d > p.skip_point) ? (d / p.skip_point) : 1);
-                const double os2 = 2.0 * p.overshoot[wg];
-                const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
-                if (q > 1 && n > 1 && lane == 0) consider<Real>(best, sentinel, k, 1);
-                const int64_t K = (n + q - 1) / q;
-                // candidates per lane: enough to cover K with one pass when possible, and
-                // (R - 1) q within the slot's padding of max_width samples
-                int32_t R = (int32_t)min((int64_t)kMaxBlock, max((int64_t)1, (K + kWave - 1) / kWave));
-                R = uniform(min(R, 1 + p.max_width / q));
-                const double ssk = p.ss[wg];
-                const int32_t F = d / q;
-                if constexpr (sizeof(Real) == 4) {
-                    if (p.use_mfma && q >= 4 && F + 1 <= 16) {
-                        // candidate k gets its window sums from the matrix-core tiles
-                        auto emit = [&](int64_t kc, float sa, float sb, double dlo, double dhi) {
-                            const int64_t s = kc * q;
-                            const double m = (dhi - dlo) / (double)d;
-                            if (m > p.depth_min) {
-                                const float rs = (float)(m * os2);
-                                const double delta =
-                                    constw ? wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)sa)
-                                           : (double)(rs * rs * sb - 2.f * rs * sa);
-                                consider<Real>(best, chi2f + delta, k, (int32_t)s);
-                            } else {
-                                consider<Real>(best, sentinel, k, (int32_t)s);
-                            }
-                        };
-                        float* ra = rings[0][wave];
-                        float* rb = rings[1][wave];
-                        if (constw)
-                            window_sums_mfma<true>((const float*)ub, np_, (const float*)(p.tmpl1 + toff),
-                                                   D, d, q, K, ra, rb, emit);
-                        else
-                            window_sums_mfma<false>((const float*)vw, np_,
-                                                    (const float*)(p.tmpl2 + toff), D, d, q, K, ra,
-                                                    rb, emit);
-                        continue;
-                    }
-                }
-                for (int64_t base = 0; base < K; base += (int64_t)kWave * R) {
-                    const int64_t k0 = base + (int64_t)lane * R;
-                    const int32_t rc = (int32_t)max((int64_t)0, min((int64_t)R, K - k0));
-                    Real A[kMaxBlock], B[kMaxBlock];
-#pragma unroll
-                    for (int r = 0; r < kMaxBlock; ++r) { A[r] = 0; B[r] = 0; }
-                    const int64_t s0 = (rc > 0 ? k0 : 0) * q;
-                    if (q == 1 && R == kMaxBlock) {
-                        if (constw)
-                            window_sums_q1<Real, true>(ub + s0, p.tmpl1 + toff, d, A, B);
-                        else
-                            window_sums_q1<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, A, B);
-                    } else {
-                        if (constw)
-                            window_sums_seg<Real, true>(ub + s0, p.tmpl1 + toff, d, q, R, A, B);
-                        else
-                            window_sums_seg<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, q, R, A, B);
-                    }
-#pragma unroll
-                    for (int r = 0; r < kMaxBlock; ++r) {
-                        if (r < rc) {
-                            const int64_t s = (k0 + r) * q;
-                            const double m = (D[s + d - 1] - (s ? D[s - 1] : 0.0)) / (double)d;
-                            if (m > p.depth_min) {
-                                const double rr = m * os2;
-                                double delta;
-                                if (constw) {
-                                    const Real rs = (Real)rr;
-                                    delta = wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)A[r]);
-                                } else {
-                                    const Real rs = (Real)rr;
-                                    delta = (double)(rs * rs * B[r] - (Real)2 * rs * A[r]);
-                                }
-                                consider<Real>(best, chi2f + delta, k, (int32_t)s);
-                            } else {
-                                consider<Real>(best, sentinel, k, (int32_t)s);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        best = block_argmin(best, red_a);
-        if (tid == 0) {
-            const bool found = best.k != 0x7fffffff;
-            p.chi2[g] = found ? best.v : __builtin_nan("");
-            p.best_k[g] = found ? best.k : -1;
-            p.best_s[g] = found ? best.s : -1;
-        }
-        }  // status == 0
-        __syncthreads();
     }
 }
 
-// Stable phase permutations only (bit-exact checks of the sort).
+// Stage 3 for one (period, width) item, by one wavefront: every candidate position
+// s = 0, q, 2q, ... (q = skip stride, GPUFun.cu:397) of width k gets the gate m > depth_min
+// on its window mean and the statistic chi2_flat + r^2 B - 2 r A; returns the wave's
+// first minimum in (width, position) order.
+template <class Real>
+__device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int64_t i,
+                                               const PeriodMeta& pm, int32_t k, float* ra,
+                                               float* rb) {
+    using R2 = typename Pair<Real>::type;
+    const int lane = lane_id();
+    const int c = pm.c;
+    const int64_t n = p.pt_off[c + 1] - p.pt_off[c];
+    const int64_t np_ = pm.np;
+    const double wc = p.wconst[c];
+    const bool constw = wc == wc;
+    const R2* vw = p.xbuf + i * p.xstride;
+    const Real* ub = (const Real*)vw;
+    const double* D = p.dbuf + i * p.xstride;
+    const double chi2f = p.chi2_flat[c];
+    const double sentinel = (double)n;
+    ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
+
+    const int64_t wg = p.wid_off[c] + k;                 // global width index
+    const int32_t d = uniform(p.widths[wg]);
+    if (d <= 0) {  // empty window: mean 0/0, every position carries the sentinel
+        if (lane == 0) consider<Real>(best, sentinel, k, 0);
+        return best;
+    }
+    const int32_t q = (int32_t)((d > p.skip_point) ? (d / p.skip_point) : 1);
+    const double os2 = 2.0 * p.overshoot[wg];
+    const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
+    if (q > 1 && n > 1 && lane == 0) consider<Real>(best, sentinel, k, 1);
+    const int64_t K = (n + q - 1) / q;
+    // candidates per lane: enough to cover K with one pass when possible, and (R - 1) q
+    // within the buffer's padding of max_width samples
+    int32_t R = (int32_t)min((int64_t)kMaxBlock, max((int64_t)1, (K + kWave - 1) / kWave));
+    R = uniform(min(R, 1 + p.max_width / q));
+    const double ssk = p.ss[wg];
+    const int32_t F = d / q;
+    if constexpr (sizeof(Real) == 4) {
+        if (p.use_mfma && q >= 4 && F + 1 <= 16) {
+            // candidate k gets its window sums from the matrix-core tiles
+            const double inv_d = 1.0 / (double)d;
+            auto emit = [&](int64_t kc, float sa, float sb, double dlo, double dhi) {
+                const int64_t s = kc * q;
+                const double m = (dhi - dlo) * inv_d;
+                if (m > p.depth_min) {
+                    const float rs = (float)(m * os2);
+                    const double delta =
+                        constw ? wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)sa)
+                               : (double)(rs * rs * sb - 2.f * rs * sa);
+                    consider<Real>(best, chi2f + delta, k, (int32_t)s);
+                } else {
+                    consider<Real>(best, sentinel, k, (int32_t)s);
+                }
+            };
+            if (constw)
+                window_sums_mfma<true>((const float*)ub, np_, (const float*)(p.tmpl1 + toff),
+                                       D, d, q, K, ra, rb, emit);
+            else
+                window_sums_mfma<false>((const float*)vw, np_, (const float*)(p.tmpl2 + toff),
+                                        D, d, q, K, ra, rb, emit);
+            return best;
+        }
+    }
+    for (int64_t base = 0; base < K; base += (int64_t)kWave * R) {
+        const int64_t k0 = base + (int64_t)lane * R;
+        const int32_t rc = (int32_t)max((int64_t)0, min((int64_t)R, K - k0));
+        Real A[kMaxBlock], B[kMaxBlock];
+#pragma unroll
+        for (int r = 0; r < kMaxBlock; ++r) { A[r] = 0; B[r] = 0; }
+        const int64_t s0 = (rc > 0 ? k0 : 0) * q;
+        if (q == 1 && R == kMaxBlock) {
+            if (constw)
+                window_sums_q1<Real, true>(ub + s0, p.tmpl1 + toff, d, A, B);
+            else
+                window_sums_q1<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, A, B);
+        } else {
+            if (constw)
+                window_sums_seg<Real, true>(ub + s0, p.tmpl1 + toff, d, q, R, A, B);
+            else
+                window_sums_seg<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, q, R, A, B);
+        }
+#pragma unroll
+        for (int r = 0; r < kMaxBlock; ++r) {
+            if (r < rc) {
+                const int64_t s = (k0 + r) * q;
+                const double m = (D[s + d - 1] - (s ? D[s - 1] : 0.0)) / (double)d;
+                if (m > p.depth_min) {
+                    const double rr = m * os2;
+                    double delta;
+                    if (constw) {
+                        const Real rs = (Real)rr;
+                        delta = wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)A[r]);
+                    } else {
+                        const Real rs = (Real)rr;
+                        delta = (double)(rs * rs * B[r] - (Real)2 * rs * A[r]);
+                    }
+                    consider<Real>(best, chi2f + delta, k, (int32_t)s);
+                } else {
+                    consider<Real>(best, sentinel, k, (int32_t)s);
+                }
+            }
+        }
+    }
+    return best;
+}
+
+// Stage 3 of the chunk: persistent wavefronts, each taking (period, width slot) items
+// from the queue of its XCD -- the periods i = x, x + 8, ... of the chunk go to XCD x, so
+// the widths of one period are searched together and its samples stay in that XCD's L2 --
+// then from the other XCDs' queues once its own is drained. Every wavefront leaves after
+// it found all eight queues empty.
+template <class Real>
+__global__ __launch_bounds__(kBlock) void windows_kernel(SearchParams<Real> p) {
+    __shared__ float rings[2 * (kBlock / kWave) * kRingFloats];
+    const int lane = lane_id();
+    const int wave = uniform((int32_t)wave_id());
+    float* ra = rings + (int64_t)wave * kRingFloats;
+    float* rb = rings + (int64_t)(kBlock / kWave + wave) * kRingFloats;
+    // One fetch per iteration and no early exits: the loop ends once all eight queues were
+    // seen empty (each queue is visited until it is, in order x0, x0 + 1, ...).
+    int x = uniform(xcd_id());
+    int drained = 0;
+    while (drained < 8) {
+        unsigned long long it = 0;
+        if (lane == 0) it = atomicAdd(&p.counters[1 + x], 1ULL);
+        const int64_t item = uniform((int64_t)it);
+        const int64_t nper = (p.chunk - x + 7) / 8;
+        const int64_t nitems = nper > 0 ? nper * p.kmax : 0;
+        if (item >= nitems) {
+            ++drained;
+            x = (x + 1) & 7;
+        } else {
+            const int64_t i = x + 8 * (item / p.kmax);
+            const int32_t j = (int32_t)(item % p.kmax);
+            PeriodMeta pm = p.meta[i];
+            pm.c = uniform(pm.c);
+            pm.status = uniform(pm.status);
+            pm.np = uniform(pm.np);
+            const int64_t g = p.p0 + i;
+            const int32_t k = uniform(p.k_lo[g] + j);
+            const int32_t khi = uniform(p.k_hi[g]);
+            if (pm.status == 0 && k <= khi) {
+                ArgMin best = search_width<Real>(p, i, pm, k, ra, rb);
+                best = wave_argmin(best);
+                if (lane == 0) p.part[i * p.kmax + j] = best;
+            }
+        }
+    }
+}
+
+// Stage 4: first minimum over each period's widths in (width, position) order
+// (cp.argmin, core.py:866-875); one thread per period of the chunk.
+template <class Real>
+__global__ void reduce_kernel(SearchParams<Real> p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.chunk) return;
+    const int64_t g = p.p0 + i;
+    const PeriodMeta pm = p.meta[i];
+    if (pm.status != 0) {
+        p.chi2[g] = __builtin_nan("");
+        p.best_k[g] = pm.status;
+        p.best_s[g] = pm.status;
+        return;
+    }
+    ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
+    const int32_t cnt = p.k_hi[g] - p.k_lo[g] + 1;
+    for (int32_t j = 0; j < cnt; ++j) {
+        const ArgMin c = p.part[i * p.kmax + j];
+        if (argmin_less(c, best)) best = c;
+    }
+    const bool found = best.k != 0x7fffffff;
+    p.chi2[g] = found ? best.v : __builtin_nan("");
+    p.best_k[g] = found ? best.k : -1;
+    p.best_s[g] = found ? best.s : -1;
+}
+
+// Stable phase permutations only (bit-exact checks of the sort). Slot layout: phases
+// (8 n), merge scratch (8 n + 4 n), bucket lists (8 kMaxBuckets).
+__host__ __device__ inline int64_t fold_sort_slot_bytes(int64_t n) {
+    return ((8 * n + 255) / 256 + (12 * n + 255) / 256 + (8 * (int64_t)kMaxBuckets + 255) / 256) * 256;
+}
+
 __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int64_t n,
                                                           const double* periods,
                                                           int64_t n_periods, int neg,
-                                                          char* slots, int64_t slot_bytes,
-                                                          int32_t* perm_out) {
-    __shared__ int hist[kMaxBuckets];
+                                                          char* slots, int32_t* perm_out) {
+    __shared__ __attribute__((aligned(16))) int hist[kMaxBuckets];
     __shared__ int red_i[kBlock / kWave];
-    char* slot = slots + (int64_t)blockIdx.x * slot_bytes;
-    double* key_in = (double*)slot;
-    double* key_out = key_in + n;
+    __shared__ int s_counts[2];
+    char* slot = slots + (int64_t)blockIdx.x * fold_sort_slot_bytes(n);
+    double* key = (double*)slot;
+    double* tmp_key = (double*)(slot + (8 * n + 255) / 256 * 256);
+    int32_t* tmp_id = (int32_t*)(tmp_key + n);
+    int64_t* lists = (int64_t*)(slot + (8 * n + 255) / 256 * 256 + (12 * n + 255) / 256 * 256);
     for (int64_t pi = blockIdx.x; pi < n_periods; pi += gridDim.x)
-        wg_fold_sort(t, n, periods[pi], neg != 0, key_in, key_out, perm_out + pi * n, hist,
-                     red_i);
+        wg_fold_sort(t, n, periods[pi], neg != 0, key, perm_out + pi * n, tmp_key, tmp_id, lists,
+                     hist, red_i, s_counts);
 }
 
 template <class Real>
@@ -424,7 +522,5 @@ __global__ __launch_bounds__(kBlock) void chi2_flat_kernel(const int64_t* pt_off
         wconst[c] = differ ? __builtin_nan("") : 1.0 / (e0 * e0);
     }
 }
-
-__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ULL; }
 
 }  // namespace gtls

@@ -148,19 +148,28 @@ __device__ __forceinline__ void window_sums_q1(const XT* __restrict__ X, const R
 namespace gtls {
 
 // ---------------------------------------------------------------------------------------
-// Matrix-core window sums (float32 search): with blocks of q samples and template chunks
-// of q samples,
+// Matrix-core window sums (float32 search). For stride q and width d = F q + rem, the
+// window of candidate k covers data blocks k .. k + F (block j = samples jq .. jq + q - 1):
 //     G(j, c) = sum_{b<q} S[c q + b] X[j q + b],      A(k) = sum_{c=0}^{F} G(k + c, c),
-// (chunk F holds the d - F q leftover template samples). G is the product of the block
-// matrix U (rows j, q columns) and the chunk matrix (F + 1 <= 16 columns): it runs on
-// v_mfma_f32_16x16x4f32, 16 blocks x 16 chunks per tile, the K loop walking b in steps
-// of 4. Operand lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result register v of lane l
-// holds C[4 (l / 16) + v][l % 16]. Finished tiles go to a 32-row LDS ring per wavefront;
-// after tile t the 16 candidates whose last block falls in it take their diagonal sums.
+// (chunk F holds the rem leftover template samples). G is the product of the block matrix
+// (rows j, q columns) and the chunk matrix (F + 1 <= 16 columns); it runs on
+// v_mfma_f32_16x16x4f32 (operand lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result
+// register v of lane l holds C[4 (l / 16) + v][l % 16]).
+//
+// A super-tile is 4 tiles of 16 G rows (rows 64 T .. 64 T + 63); the 4 tiles share the
+// chunk operand. Finished rows go to an 80-row LDS ring per wavefront (64 new rows plus
+// the F <= 15 rows of the previous super-tile that its last candidates still need); then
+// each lane takes the diagonal sum of one candidate, k = 64 T - F + lane.
+//
+// The K (b) loop runs in groups of 4 M samples = M MFMA steps; within a group, step s uses
+// sample b0 + M (l / 16) + s in K slot l / 16, so each lane's operands for a group are M
+// consecutive floats. The loads of the next group are issued before the MFMAs of the
+// current one.
 // ---------------------------------------------------------------------------------------
 typedef float mfma_v4 __attribute__((ext_vector_type(4)));
-constexpr int kRingRows = 32;
+constexpr int kRingRows = 80;     // 64 rows of a super-tile + up to 15 look-ahead rows
 constexpr int kRingStride = 17;   // 16 chunk columns + 1 (bank spread)
+constexpr int kRingFloats = kRingRows * kRingStride;
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -168,130 +177,184 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Loads four consecutive floats (16 B) of a sample or template stream.
-__device__ __forceinline__ float4 ld4(const float* p) {
-    float4 v;
-    __builtin_memcpy(&v, p, sizeof(v));
-    return v;
+template <int M>
+struct FV {
+    float v[M];
+};
+
+// M consecutive floats (one 4 M-byte load).
+template <int M>
+__device__ __forceinline__ FV<M> ldm(const float* p) {
+    FV<M> r;
+    __builtin_memcpy(&r, p, sizeof(r));
+    return r;
 }
 
-__device__ __forceinline__ float4 mask4(float4 v, int32_t b, int32_t lim) {
-    v.x = (b + 0 < lim) ? v.x : 0.f;
-    v.y = (b + 1 < lim) ? v.y : 0.f;
-    v.z = (b + 2 < lim) ? v.z : 0.f;
-    v.w = (b + 3 < lim) ? v.w : 0.f;
-    return v;
+// 2 M consecutive floats split into even (first) and odd (second) elements.
+template <int M>
+__device__ __forceinline__ void ldm2(const float* p, FV<M>& e, FV<M>& o) {
+    FV<2 * M> r;
+    __builtin_memcpy(&r, p, sizeof(r));
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        e.v[i] = r.v[2 * i];
+        o.v[i] = r.v[2 * i + 1];
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void maskm(FV<M>& x, int32_t b, int32_t lim) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) x.v[i] = (b + i < lim) ? x.v[i] : 0.f;
+}
+
+__device__ __forceinline__ int ring_wrap(int r) {
+    r = r < 0 ? r + kRingRows : r;
+    return r >= kRingRows ? r - kRingRows : r;
+}
+
+// MFMA steps of stride q with groups of 4 M samples.
+__host__ __device__ constexpr int32_t mfma_steps(int32_t q, int32_t m) {
+    return m * ((q + 4 * m - 1) / (4 * m));
 }
 
 // X: the item's gathered samples (u, or interleaved (v, w) when !ConstW); samples at or
-// past np are treated as 0 (they are read but never used unmasked).
-// T: the width's template (S, or interleaved (S, S^2)); reads past the row land in
-// following rows or the zeroed tail of the template buffer and are masked.
-// D: prefix sums of u (float64), used for the candidates' window means.
+// past np are treated as 0 (read, never used unmasked; reads stay below
+// np + d + q + 16). T: the width's template (S, or interleaved (S, S^2)); reads past the
+// row (at most q + 15 samples) land in following rows or the zeroed tail of the template
+// buffer and are masked. D: prefix sums of u (float64) for the window means.
 // emit(k, A, B, Dlo, Dhi) is called by one lane per candidate k < K.
-//
-// The K (b) loop runs in groups of 16 samples = 4 MFMA steps; within a group, step s uses
-// sample b0 + 4 (l / 16) + s in K slot l / 16, so each lane's operands for the group are 4
-// consecutive floats (one 16-byte load per stream). The loads of the next group are issued
-// before the MFMAs of the current one.
-template <bool ConstW, class Emit>
-__device__ __forceinline__ void window_sums_mfma(const float* __restrict__ X, int64_t np,
-                                                 const float* __restrict__ T,
-                                                 const double* __restrict__ D, int32_t d,
-                                                 int32_t q, int64_t K, float* ringA,
-                                                 float* ringB, Emit emit) {
+template <bool ConstW, int M, class Emit>
+__device__ __forceinline__ void window_sums_mfma_m(const float* __restrict__ X, int64_t np,
+                                                   const float* __restrict__ T,
+                                                   const double* __restrict__ D, int32_t d,
+                                                   int32_t q, int64_t K, float* ringA,
+                                                   float* ringB, Emit emit) {
+    constexpr int GS = 4 * M;
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
     const int32_t F = d / q, rem = d - F * q;
     const int64_t rows = K + F;
-    const int64_t ntiles = (rows + 15) / 16;
+    const int64_t nsuper = (rows + 63) / 64;
     const int32_t clen = (col < F) ? q : (col == F ? rem : 0);   // valid b of this chunk
-    // columns past the last chunk read row 0 (masked): keeps reads within d + q + 16
+    // columns past the last chunk read chunk 0 (masked)
     const int64_t tcol = (col <= F) ? (int64_t)col * q : 0;
-    const int32_t ngroups = (q + 15) / 16;
-    for (int64_t t = 0; t < ntiles; ++t) {
-        // the window-mean prefix sums of this tile's candidates, requested early
-        const int64_t kc = 16 * t - F + lane;
-        const bool cand = lane < 16 && kc >= 0 && kc < K;
+    const int32_t ngroups = (q + GS - 1) / GS;
+    int ring0 = 0;   // ring row of G row 64 T
+    for (int64_t st = 0; st < nsuper; ++st) {
+        // the window-mean prefix sums of this super-tile's candidates, requested early
+        const int64_t kc = 64 * st - F + lane;
+        const bool cand = kc >= 0 && kc < K;
         double dlo = 0.0, dhi = 0.0;
         if (cand) {
             const int64_t s = kc * q;
             dhi = D[s + d - 1];
             dlo = s ? D[s - 1] : 0.0;
         }
-        const int64_t row = 16 * t + col;
-        const bool rowok = row < rows;
-        // rows past the last read block 0 (masked): keeps reads below n + d + q + 16
-        const int64_t abase = rowok ? row * q : 0;
-        mfma_v4 acc = {0.f, 0.f, 0.f, 0.f};
-        mfma_v4 acc2 = {0.f, 0.f, 0.f, 0.f};
-        // group g covers samples [16 g, 16 g + 16) of every block; lane part: 4 (l/16)
-        auto load_group = [&](int32_t g, float4& a, float4& a2, float4& bv, float4& bv2) {
-            const int32_t b = 16 * g + 4 * kk;
-            const int64_t ai = abase + b;
+        int64_t abase[4];
+        int32_t alim[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int64_t row = 64 * st + 16 * t + col;
+            const bool ok = row < rows;
+            // rows past the last read block 0 (masked)
+            abase[t] = ok ? row * q : 0;
+            alim[t] = ok ? (int32_t)min((int64_t)q, max((int64_t)0, np - row * q)) : 0;
+        }
+        mfma_v4 acc[4], acc2[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+            acc2[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+        }
+        FV<M> a[4], a2[4], bv, bv2;
+        auto load_group = [&](int32_t g, FV<M>(&xa)[4], FV<M>(&xa2)[4], FV<M>& xb,
+                              FV<M>& xb2) {
+            const int32_t b = GS * g + M * kk;
             if constexpr (ConstW) {
-                a = ld4(X + ai);
-                bv = ld4(T + tcol + b);
+                xb = ldm<M>(T + tcol + b);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) xa[t] = ldm<M>(X + abase[t] + b);
             } else {
-                const float4 p0 = ld4(X + 2 * ai), p1 = ld4(X + 2 * ai + 4);
-                a = make_float4(p0.x, p0.z, p1.x, p1.z);
-                a2 = make_float4(p0.y, p0.w, p1.y, p1.w);
-                const float4 q0 = ld4(T + 2 * (tcol + b)), q1 = ld4(T + 2 * (tcol + b) + 4);
-                bv = make_float4(q0.x, q0.z, q1.x, q1.z);
-                bv2 = make_float4(q0.y, q0.w, q1.y, q1.w);
+                ldm2<M>(T + 2 * (tcol + b), xb, xb2);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) ldm2<M>(X + 2 * (abase[t] + b), xa[t], xa2[t]);
             }
-            // A: samples past the block (b >= q) or past the data are zero; B: past the
-            // chunk's length
-            const int32_t alim = rowok ? (int32_t)min((int64_t)q, max((int64_t)0, np - abase)) : 0;
-            a = mask4(a, b, alim);
-            bv = mask4(bv, b, clen);
-            if constexpr (!ConstW) {
-                a2 = mask4(a2, b, alim);
-                bv2 = mask4(bv2, b, clen);
+            maskm<M>(xb, b, clen);
+            if constexpr (!ConstW) maskm<M>(xb2, b, clen);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                maskm<M>(xa[t], b, alim[t]);
+                if constexpr (!ConstW) maskm<M>(xa2[t], b, alim[t]);
             }
         };
-        float4 a, a2, bv, bv2;
         load_group(0, a, a2, bv, bv2);
         for (int32_t g = 0; g < ngroups; ++g) {
-            float4 na, na2, nb, nb2;
-            if (g + 1 < ngroups) load_group(g + 1, na, na2, nb, nb2);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bv.x, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bv.y, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, bv.z, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, bv.w, acc, 0, 0, 0);
-            if constexpr (!ConstW) {
-                acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2.x, bv2.x, acc2, 0, 0, 0);
-                acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2.y, bv2.y, acc2, 0, 0, 0);
-                acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2.z, bv2.z, acc2, 0, 0, 0);
-                acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2.w, bv2.w, acc2, 0, 0, 0);
-            }
-            if (g + 1 < ngroups) {
-                a = na;
-                bv = nb;
-                if constexpr (!ConstW) {
-                    a2 = na2;
-                    bv2 = nb2;
+            FV<M> na[4], na2[4], nb, nb2;
+            const bool more = g + 1 < ngroups;
+            if (more) load_group(g + 1, na, na2, nb, nb2);
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t].v[i], bv.v[i], acc[t],
+                                                                  0, 0, 0);
+                    if constexpr (!ConstW)
+                        acc2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2[t].v[i], bv2.v[i],
+                                                                       acc2[t], 0, 0, 0);
                 }
+            }
+            if (more) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    a[t] = na[t];
+                    if constexpr (!ConstW) a2[t] = na2[t];
+                }
+                bv = nb;
+                if constexpr (!ConstW) bv2 = nb2;
             }
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = (int)((16 * t + 4 * kk + v) & (kRingRows - 1));
-            ringA[r * kRingStride + col] = acc[v];
-            if constexpr (!ConstW) ringB[r * kRingStride + col] = acc2[v];
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int r = ring0 + 16 * t + 4 * kk + v;
+                r = r >= kRingRows ? r - kRingRows : r;
+                ringA[r * kRingStride + col] = acc[t][v];
+                if constexpr (!ConstW) ringB[r * kRingStride + col] = acc2[t][v];
+            }
         }
         wave_sync_lds();
         if (cand) {
             float sa = 0.f, sb = 0.f;
+            const int r0 = ring0 - F + lane;
             for (int32_t c = 0; c <= F; ++c) {
-                const int r = (int)((kc + c) & (kRingRows - 1));
+                const int r = ring_wrap(r0 + c);
                 sa += ringA[r * kRingStride + c];
                 if constexpr (!ConstW) sb += ringB[r * kRingStride + c];
             }
             emit(kc, sa, sb, dlo, dhi);
         }
         wave_sync_lds();
+        ring0 = ring0 + 64 >= kRingRows ? ring0 + 64 - kRingRows : ring0 + 64;
     }
+}
+
+// Picks the group size (4, 8 or 16 samples) with the fewest MFMA steps for stride q,
+// preferring larger groups (fewer, wider loads) on ties.
+template <bool ConstW, class Emit>
+__device__ __forceinline__ void window_sums_mfma(const float* __restrict__ X, int64_t np,
+                                                 const float* __restrict__ T,
+                                                 const double* __restrict__ D, int32_t d,
+                                                 int32_t q, int64_t K, float* ringA,
+                                                 float* ringB, Emit emit) {
+    const int32_t s1 = mfma_steps(q, 1), s2 = mfma_steps(q, 2), s4 = mfma_steps(q, 4);
+    if (s4 <= s1 && s4 <= s2)
+        window_sums_mfma_m<ConstW, 4>(X, np, T, D, d, q, K, ringA, ringB, emit);
+    else if (s2 <= s1)
+        window_sums_mfma_m<ConstW, 2>(X, np, T, D, d, q, K, ringA, ringB, emit);
+    else
+        window_sums_mfma_m<ConstW, 1>(X, np, T, D, d, q, K, ringA, ringB, emit);
 }
 
 }  // namespace gtls

@@ -84,6 +84,8 @@ typedef struct gtls_batch {
     int64_t total_tmpl;
     int32_t max_points;      /* max over curves of the point count */
     int32_t max_width;       /* max over all widths */
+    int32_t max_period_widths; /* max over periods of k_hi - k_lo + 1 (an upper bound is
+                                  fine, e.g. the largest width table of a curve) */
 } gtls_batch;
 
 /* Context bound to one device and one stream (hipStream_t, may be NULL). */

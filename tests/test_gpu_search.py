@@ -74,6 +74,12 @@ def test_fold_sort_bit_exact(gpu):
         (1000 + np.arange(20000) / 720.0, np.array([0.5, 1.0, 1.5, 2.0, 3.71, 7.0])),  # ties
         (rng.uniform(-40, 40, 3000), rng.uniform(0.4, 9, 20)),                        # t < 0
         (1000 + np.arange(300) / 48.0, np.array([0.5, 0.75, 1.0])),                   # small n
+        # periods that are multiples of the cadence: ~139 and ~69 equal phases per bucket
+        (1000 + np.arange(50000) / 720.0, np.array([0.5, 0.25, 0.5 + 1e-9])),
+        # 8 distinct times repeated 600 times: 600-key buckets
+        (np.repeat(1000 + np.arange(8) / 48.0, 600), np.array([0.5, 0.77])),
+        # one time repeated: a single bucket of 5000 equal phases, ordered by index
+        (np.full(5000, 1234.5), np.array([0.7, 3.1])),
     ]
     for t, periods in cases:
         got = S.fold_sort(t, periods).cpu().numpy()
