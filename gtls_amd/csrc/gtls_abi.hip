@@ -55,7 +55,8 @@ int hip_fail(gtls_ctx* ctx, hipError_t e, const char* where) {
 }
 
 struct SearchLayout {
-    int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, ss = 0, wconst = 0, flat = 0, counters = 0;
+    int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, tmplsq = 0, ss = 0, wconst = 0, flat = 0, counters = 0;
+    int64_t tmplc1 = 0, tmplc2 = 0, tzero = 0, n_chunked = 0, n_zero = 0;
     int64_t slots = 0, slot_bytes = 0, o_key = 0, o_perm = 0, o_tmp_key = 0, o_tmp_id = 0;
     int64_t o_lists = 0, n_slots = 0;
     int64_t xbuf = 0, dbuf = 0, meta = 0, part = 0;
@@ -79,6 +80,19 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     off = align_up(off + 2 * real * tmpl_padded);
     L.tmpl1 = off;
     off = align_up(off + real * tmpl_padded);
+    L.tmplsq = off;
+    off = align_up(off + real * tmpl_padded);
+    // float32: chunked rows of the matrix-core path and a block of zeros
+    if (mode == GTLS_MODE_FP32C) {
+        L.n_chunked = 2 * b->total_tmpl + 1024 * b->total_widths + 1024;
+        L.n_zero = max_width + 128;
+    }
+    L.tmplc1 = off;
+    off = align_up(off + 4 * L.n_chunked);
+    L.tmplc2 = off;
+    off = align_up(off + 4 * L.n_chunked);
+    L.tzero = off;
+    off = align_up(off + 4 * L.n_zero);
     L.ss = off;
     off = align_up(off + 8 * (b->total_widths > 0 ? b->total_widths : 1));
     L.wconst = off;
@@ -104,7 +118,7 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     off = align_up(off + L.slot_bytes * L.n_slots);
     // per period of a chunk: gathered samples N + widest width of the wrap, plus up to
     // another max_width read (and discarded) past them by the window loops
-    L.xstride = max_points + 2 * max_width + 64;
+    L.xstride = max_points + 2 * max_width + 128;
     L.kmax = b->max_period_widths;
     const int64_t per_period =
         L.xstride * (2 * real + 8) + (int64_t)sizeof(PeriodMeta) + L.kmax * (int64_t)sizeof(ArgMin);
@@ -168,6 +182,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     R2* uw0 = (R2*)(ws + L.uw0);
     R2* tmpl2 = (R2*)(ws + L.tmpl2);
     Real* tmpl1 = (Real*)(ws + L.tmpl1);
+    Real* tmplsq = (Real*)(ws + L.tmplsq);
     double* ss = (double*)(ws + L.ss);
     double* wconst = (double*)(ws + L.wconst);
     double* flat = (double*)(ws + L.flat);
@@ -178,7 +193,12 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
                        dim3(tpb), 0, st, b->y, b->dy, b->total_points, uw0);
     hipLaunchKernelGGL(prep_template_kernel<Real>, dim3((unsigned)b->total_widths + 8), dim3(tpb),
                        0, st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->total_tmpl, tmpl2,
-                       tmpl1);
+                       tmpl1, tmplsq);
+    if (L.n_chunked > 0)
+        hipLaunchKernelGGL(chunk_template_kernel, dim3((unsigned)b->total_widths + 4), dim3(tpb), 0,
+                           st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->skip_point,
+                           (float*)(ws + L.tmplc1), (float*)(ws + L.tmplc2),
+                           (float*)(ws + L.tzero), L.n_zero);
     hipLaunchKernelGGL(template_ss_kernel<Real>, dim3((unsigned)b->total_widths), dim3(kWave), 0,
                        st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, ss);
     hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
@@ -202,6 +222,10 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.uw0 = uw0;
     p.tmpl2 = tmpl2;
     p.tmpl1 = tmpl1;
+    p.tmplsq = tmplsq;
+    p.tmplc1 = (const float*)(ws + L.tmplc1);
+    p.tmplc2 = (const float*)(ws + L.tmplc2);
+    p.tzero = (const float*)(ws + L.tzero);
     p.ss = ss;
     p.chi2_flat = flat;
     p.wconst = wconst;

@@ -26,9 +26,6 @@ constexpr int kBlock = 256;
 constexpr int64_t kTmplTail = 32768;  // zeroed samples after the last template row
 constexpr int kScanPerThread = 8;  // gathered elements per thread per prefix-scan chunk
 
-template <class Real> struct Pair;
-template <> struct Pair<float> { using type = float2; };
-template <> struct Pair<double> { using type = double2; };
 
 // Per trial period of a chunk: written by prep_kernel, read by the two later stages.
 struct PeriodMeta {
@@ -54,6 +51,11 @@ struct SearchParams {
     const R2* uw0;          // per point (u, w)
     const R2* tmpl2;        // per template sample (S, S^2)
     const Real* tmpl1;      // per template sample S
+    const Real* tmplsq;     // per template sample S^2
+    // float32: chunked template rows of the matrix-core path (S, S^2) and a block of zeros
+    const float* tmplc1;
+    const float* tmplc2;
+    const float* tzero;
     const double* ss;       // per width: sum of S^2 over the row
     const double* chi2_flat;  // per curve
     const double* wconst;   // per curve: the common weight 1/dy^2, or NaN if dy varies
@@ -71,8 +73,9 @@ struct SearchParams {
     char* slots;
     int64_t slot_bytes;
     int64_t o_key, o_perm, o_tmp_key, o_tmp_id, o_lists;
-    // per chunk period i: gathered samples xbuf[i * xstride ...] (u, or (u w, w) pairs)
-    // and their float64 running sum dbuf[i * xstride ...]
+    // per chunk period i: gathered samples in xbuf[i * xstride ...] (room for 2 xstride
+    // Reals: u, or the planes v = u w at [0, xstride) and w at [xstride, 2 xstride)) and
+    // their float64 running sum dbuf[i * xstride ...]
     R2* xbuf;
     double* dbuf;
     int64_t xstride;
@@ -94,11 +97,6 @@ __device__ __forceinline__ int find_owner(const int64_t* off, int n, int64_t g) 
     return lo;
 }
 
-template <class Real>
-__device__ __forceinline__ void consider(ArgMin& b, double v, int32_t k, int32_t s) {
-    ArgMin c{v, k, s};
-    if (argmin_less(c, b)) b = c;
-}
 
 // The XCD (accelerator complex die) this wavefront runs on, 0..7.
 __device__ __forceinline__ int xcd_id() {
@@ -162,11 +160,12 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
                          lists, hist, red_i, s_counts);
 
             // gather in phase order (wrapping) and the float64 running sum of u. With a
-            // common weight only u is kept (X = u), otherwise the pairs (u w, w).
+            // common weight only u is kept (X = u), otherwise the planes v = u w and w.
             const double wc = p.wconst[c];
             const bool constw = wc == wc;
-            R2* vw = p.xbuf + i * p.xstride;
-            Real* ub = (Real*)vw;
+            Real* ub = (Real*)(p.xbuf + i * p.xstride);
+            Real* vpl = ub;
+            Real* wpl = ub + p.xstride;
             double* D = p.dbuf + i * p.xstride;
             const int lane = lane_id(), wave = wave_id();
             const int64_t chunk = (int64_t)kBlock * kScanPerThread;
@@ -187,10 +186,8 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
                         if (constw) {
                             ub[j] = uu;
                         } else {
-                            R2 o;
-                            o.x = uu * ww;
-                            o.y = ww;
-                            vw[j] = o;
+                            vpl[j] = uu * ww;
+                            wpl[j] = ww;
                         }
                     }
                     const double s = wave_inclusive_scan((double)uu);
@@ -212,15 +209,62 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
                 }
                 carry = total;
             }
-            // finite values past the sequence: the stride-1 window path multiplies them
-            // by zero template padding
-            if (tid < kTmplPad) {
-                if (constw) ub[np_ + tid] = (Real)0;
-                else vw[np_ + tid] = R2{(Real)0, (Real)0};
+            // zeros past the sequence, up to where the window loops read (below
+            // N + d + q + 64 <= N + 2 dmax + 64): those samples meet zero template entries
+            const int64_t zend = min(p.xstride, n + 2 * (int64_t)dmax + 96);
+            for (int64_t j = np_ + tid; j < zend; j += kBlock) {
+                if (constw) {
+                    ub[j] = (Real)0;
+                } else {
+                    vpl[j] = (Real)0;
+                    wpl[j] = (Real)0;
+                }
             }
         }
         __syncthreads();
     }
+}
+
+// Register-blocked VALU windows of one (period, width) item (strides the matrix-core path
+// does not take, and the float64 mode), returning the lane's first minimum.
+template <class Real, bool ConstW, class XS>
+__device__ __forceinline__ ArgMin valu_width(XS X, const Real* T, const double* __restrict__ D,
+                                         WidthArgs a, int32_t R) {
+    const int lane = lane_id();
+    const int32_t d = a.d, q = a.q;
+    const int64_t K = a.K;
+    ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
+    for (int64_t base = 0; base < K; base += (int64_t)kWave * R) {
+        const int64_t k0 = base + (int64_t)lane * R;
+        const int32_t rc = (int32_t)max((int64_t)0, min((int64_t)R, K - k0));
+        Real A[kMaxBlock], B[kMaxBlock];
+#pragma unroll
+        for (int r = 0; r < kMaxBlock; ++r) { A[r] = 0; B[r] = 0; }
+        const int64_t s0 = (rc > 0 ? k0 : 0) * q;
+        if (q == 1 && R == kMaxBlock)
+            window_sums_q1<Real, ConstW>(X + s0, T, d, A, B);
+        else
+            window_sums_seg<Real, ConstW>(X + s0, T, d, q, R, A, B);
+#pragma unroll
+        for (int r = 0; r < kMaxBlock; ++r) {
+            if (r < rc) {
+                const int64_t s = (k0 + r) * q;
+                const double m = (D[s + d - 1] - (s ? D[s - 1] : 0.0)) / (double)d;
+                if (m > a.depth_min) {
+                    const Real rs = (Real)(m * a.os2);
+                    double delta;
+                    if constexpr (ConstW)
+                        delta = a.wc * ((double)(rs * rs) * a.ssk - 2.0 * (double)rs * (double)A[r]);
+                    else
+                        delta = (double)(rs * rs * B[r] - (Real)2 * rs * A[r]);
+                    consider(best, a.chi2f + delta, a.k, (int32_t)s);
+                } else {
+                    consider(best, a.sentinel, a.k, (int32_t)s);
+                }
+            }
+        }
+    }
+    return best;
 }
 
 // Stage 3 for one (period, width) item, by one wavefront: every candidate position
@@ -229,105 +273,62 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
 // first minimum in (width, position) order.
 template <class Real>
 __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int64_t i,
-                                               const PeriodMeta& pm, int32_t k, float* ra,
-                                               float* rb) {
-    using R2 = typename Pair<Real>::type;
+                                               const PeriodMeta& pm, int32_t k) {
     const int lane = lane_id();
     const int c = pm.c;
     const int64_t n = p.pt_off[c + 1] - p.pt_off[c];
-    const int64_t np_ = pm.np;
     const double wc = p.wconst[c];
     const bool constw = wc == wc;
-    const R2* vw = p.xbuf + i * p.xstride;
-    const Real* ub = (const Real*)vw;
+    const Real* ub = (const Real*)(p.xbuf + i * p.xstride);
+    const Planar<Real> vw{ub, ub + p.xstride};
     const double* D = p.dbuf + i * p.xstride;
-    const double chi2f = p.chi2_flat[c];
-    const double sentinel = (double)n;
     ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
 
     const int64_t wg = p.wid_off[c] + k;                 // global width index
     const int32_t d = uniform(p.widths[wg]);
+    WidthArgs a;
+    a.chi2f = p.chi2_flat[c];
+    a.sentinel = (double)n;
+    a.wc = wc;
+    a.depth_min = p.depth_min;
+    a.k = k;
     if (d <= 0) {  // empty window: mean 0/0, every position carries the sentinel
-        if (lane == 0) consider<Real>(best, sentinel, k, 0);
+        if (lane == 0) consider(best, a.sentinel, k, 0);
         return best;
     }
     const int32_t q = (int32_t)((d > p.skip_point) ? (d / p.skip_point) : 1);
-    const double os2 = 2.0 * p.overshoot[wg];
-    const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
-    if (q > 1 && n > 1 && lane == 0) consider<Real>(best, sentinel, k, 1);
-    const int64_t K = (n + q - 1) / q;
-    // candidates per lane: enough to cover K with one pass when possible, and (R - 1) q
-    // within the buffer's padding of max_width samples
-    int32_t R = (int32_t)min((int64_t)kMaxBlock, max((int64_t)1, (K + kWave - 1) / kWave));
-    R = uniform(min(R, 1 + p.max_width / q));
-    const double ssk = p.ss[wg];
+    a.d = d;
+    a.q = q;
+    a.os2 = 2.0 * p.overshoot[wg];
+    a.ssk = p.ss[wg];
+    a.inv_d = 1.0 / (double)d;
+    a.K = (n + q - 1) / q;
+    if (q > 1 && n > 1 && lane == 0) consider(best, a.sentinel, k, 1);
     const int32_t F = d / q;
+    ArgMin w;
     if constexpr (sizeof(Real) == 4) {
         if (p.use_mfma && q >= 4 && F + 1 <= 16) {
-            // candidate k gets its window sums from the matrix-core tiles
-            const double inv_d = 1.0 / (double)d;
-            auto emit = [&](int64_t kc, float sa, float sb, double dlo, double dhi) {
-                const int64_t s = kc * q;
-                const double m = (dhi - dlo) * inv_d;
-                if (m > p.depth_min) {
-                    const float rs = (float)(m * os2);
-                    const double delta =
-                        constw ? wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)sa)
-                               : (double)(rs * rs * sb - 2.f * rs * sa);
-                    consider<Real>(best, chi2f + delta, k, (int32_t)s);
-                } else {
-                    consider<Real>(best, sentinel, k, (int32_t)s);
-                }
-            };
+            const int64_t co = uniform(chunk_offset(p.tmpl_off[wg], wg));
             if (constw)
-                window_sums_mfma<true>((const float*)ub, np_, (const float*)(p.tmpl1 + toff),
-                                       D, d, q, K, ra, rb, emit);
+                w = window_sums_mfma<true>((const float*)ub, nullptr, p.tmplc1 + co, nullptr,
+                                           p.tzero, D, a);
             else
-                window_sums_mfma<false>((const float*)vw, np_, (const float*)(p.tmpl2 + toff),
-                                        D, d, q, K, ra, rb, emit);
+                w = window_sums_mfma<false>((const float*)vw.v, (const float*)vw.w,
+                                            p.tmplc1 + co, p.tmplc2 + co, p.tzero, D, a);
+            if (argmin_less(w, best)) best = w;
             return best;
         }
     }
-    for (int64_t base = 0; base < K; base += (int64_t)kWave * R) {
-        const int64_t k0 = base + (int64_t)lane * R;
-        const int32_t rc = (int32_t)max((int64_t)0, min((int64_t)R, K - k0));
-        Real A[kMaxBlock], B[kMaxBlock];
-#pragma unroll
-        for (int r = 0; r < kMaxBlock; ++r) { A[r] = 0; B[r] = 0; }
-        const int64_t s0 = (rc > 0 ? k0 : 0) * q;
-        if (q == 1 && R == kMaxBlock) {
-            if (constw)
-                window_sums_q1<Real, true>(ub + s0, p.tmpl1 + toff, d, A, B);
-            else
-                window_sums_q1<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, A, B);
-        } else {
-            if (constw)
-                window_sums_seg<Real, true>(ub + s0, p.tmpl1 + toff, d, q, R, A, B);
-            else
-                window_sums_seg<Real, false>(vw + s0, (const Real*)(p.tmpl2 + toff), d, q, R, A, B);
-        }
-#pragma unroll
-        for (int r = 0; r < kMaxBlock; ++r) {
-            if (r < rc) {
-                const int64_t s = (k0 + r) * q;
-                const double m = (D[s + d - 1] - (s ? D[s - 1] : 0.0)) / (double)d;
-                if (m > p.depth_min) {
-                    const double rr = m * os2;
-                    double delta;
-                    if (constw) {
-                        const Real rs = (Real)rr;
-                        delta = wc * ((double)(rs * rs) * ssk - 2.0 * (double)rs * (double)A[r]);
-                    } else {
-                        const Real rs = (Real)rr;
-                        delta = (double)(rs * rs * B[r] - (Real)2 * rs * A[r]);
-                    }
-                    consider<Real>(best, chi2f + delta, k, (int32_t)s);
-                } else {
-                    consider<Real>(best, sentinel, k, (int32_t)s);
-                }
-            }
-        }
-    }
+    const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
+    // candidates per lane: enough to cover K with one pass when possible, and (R - 1) q
+    // within the buffer's padding of max_width samples
+    int32_t R = (int32_t)min((int64_t)kMaxBlock, max((int64_t)1, (a.K + kWave - 1) / kWave));
+    R = uniform(min(R, 1 + p.max_width / q));
+    if (constw)
+        w = valu_width<Real, true>(ub, p.tmpl1 + toff, D, a, R);
+    else
+        w = valu_width<Real, false>(vw, (const Real*)(p.tmpl2 + toff), D, a, R);
+    if (argmin_less(w, best)) best = w;
     return best;
 }
 
@@ -338,11 +339,7 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
 // it found all eight queues empty.
 template <class Real>
 __global__ __launch_bounds__(kBlock) void windows_kernel(SearchParams<Real> p) {
-    __shared__ float rings[2 * (kBlock / kWave) * kRingFloats];
     const int lane = lane_id();
-    const int wave = uniform((int32_t)wave_id());
-    float* ra = rings + (int64_t)wave * kRingFloats;
-    float* rb = rings + (int64_t)(kBlock / kWave + wave) * kRingFloats;
     // One fetch per iteration and no early exits: the loop ends once all eight queues were
     // seen empty (each queue is visited until it is, in order x0, x0 + 1, ...).
     int x = uniform(xcd_id());
@@ -367,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void windows_kernel(SearchParams<Real> p) {
             const int32_t k = uniform(p.k_lo[g] + j);
             const int32_t khi = uniform(p.k_hi[g]);
             if (pm.status == 0 && k <= khi) {
-                ArgMin best = search_width<Real>(p, i, pm, k, ra, rb);
+                ArgMin best = search_width<Real>(p, i, pm, k);
                 best = wave_argmin(best);
                 if (lane == 0) p.part[i * p.kmax + j] = best;
             }
@@ -438,12 +435,12 @@ __global__ void prep_points_kernel(const double* y, const double* dy, int64_t n,
     uw[i] = o;
 }
 
-// Search-precision template rows, (S, S^2) pairs and S alone, each row surrounded by
+// Search-precision template rows, (S, S^2) pairs, S alone and S^2 alone, each row surrounded by
 // kTmplPad zeros: row w starts at tmpl_off[w] + 2 kTmplPad w + kTmplPad. One block per row.
 template <class Real>
 __global__ void prep_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
                                      int64_t nw, int64_t total_tmpl,
-                                     typename Pair<Real>::type* out, Real* out1) {
+                                     typename Pair<Real>::type* out, Real* out1, Real* outsq) {
     const int64_t w = blockIdx.x;
     if (w >= nw) {
         // the zeroed tail after the last row (over-reads of the matrix-core path)
@@ -455,6 +452,7 @@ __global__ void prep_template_kernel(const double* S, const int64_t* off, const 
             o.y = (Real)0;
             out[i] = o;
             out1[i] = (Real)0;
+            outsq[i] = (Real)0;
         }
         return;
     }
@@ -469,6 +467,37 @@ __global__ void prep_template_kernel(const double* S, const int64_t* off, const 
         o.y = (Real)(s * s);
         out[dst + i] = o;
         out1[dst + i] = (Real)s;
+        outsq[dst + i] = (Real)(s * s);
+    }
+}
+
+// Chunked template rows of the matrix-core path (float32): one block per width; widths the
+// matrix-core path does not take (q < 4 or more than 16 chunks) get no row. The extra
+// blocks past the last width clear the zero block (nz floats).
+__global__ void chunk_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
+                                      int64_t nw, int64_t skip_point, float* c1, float* c2,
+                                      float* zero, int64_t nz) {
+    const int64_t w = blockIdx.x;
+    if (w >= nw) {
+        for (int64_t i = (int64_t)(w - nw) * blockDim.x + threadIdx.x; i < nz;
+             i += (int64_t)(gridDim.x - nw) * blockDim.x)
+            zero[i] = 0.f;
+        return;
+    }
+    const int32_t d = widths[w];
+    if (d <= 0) return;
+    const int32_t q = (int32_t)((d > skip_point) ? (d / skip_point) : 1);
+    const int32_t F = d / q, rem = d - F * q;
+    if (q < 4 || F + 1 > 16) return;
+    const int32_t qp = chunk_pad(q);
+    const int64_t co = chunk_offset(off[w], w);
+    const int32_t total = (F + 1) * qp;
+    for (int32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        const int32_t c = i / qp, b = i - c * qp;
+        const int32_t len = c < F ? q : rem;
+        const double s = b < len ? S[off[w] + (int64_t)c * q + b] : 0.0;
+        c1[co + i] = (float)s;
+        c2[co + i] = (float)(s * s);
     }
 }
 

@@ -14,11 +14,31 @@
 // Templates are stored with kTmplPad zero samples on both sides of each row, so the
 // stride-1 path can read a fixed 15-sample span without bounds tests.
 #pragma once
+#include <type_traits>
 #include "wg_primitives.h"
 
 namespace gtls {
 
+template <class Real> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+
 constexpr int kMaxBlock = 8;   // candidates per lane
+
+// Wave-uniform values of one (period, width) item that the window routines need to turn
+// window sums into the statistic: chi2_flat, the sentinel N, the common weight, sum S^2,
+// 2 overshoot, 1 / d, the depth gate; the number of candidates K, the width d, its
+// stride q and its index k.
+struct WidthArgs {
+    double chi2f, sentinel, wc, ssk, os2, inv_d, depth_min;
+    int64_t K;
+    int32_t d, q, k;
+};
+
+__device__ __forceinline__ void consider(ArgMin& b, double v, int32_t k, int32_t s) {
+    ArgMin c{v, k, s};
+    if (argmin_less(c, b)) b = c;
+}
 constexpr int kTmplPad = 16;   // zero samples before and after each template row
 
 // Template rows are read through the constant address space: they are never written by
@@ -44,6 +64,20 @@ __device__ __forceinline__ TV<Real> tload(const_ptr<Real> T, int32_t i) {
     else return TV<Real>{T[2 * i], T[2 * i + 1]};
 }
 
+// Samples of the weighted path in two planes (v = u w and w), read as (v, w) pairs.
+template <class Real>
+struct Planar {
+    const Real* v;
+    const Real* w;
+    __device__ __forceinline__ typename Pair<Real>::type operator[](int64_t i) const {
+        typename Pair<Real>::type r;
+        r.x = v[i];
+        r.y = w[i];
+        return r;
+    }
+    __device__ __forceinline__ Planar operator+(int64_t o) const { return Planar{v + o, w + o}; }
+};
+
 template <bool ConstW, class Real, class XT>
 __device__ __forceinline__ void acc1(Real& A, Real& B, const TV<Real> t, const XT x) {
     if constexpr (ConstW) {
@@ -58,9 +92,10 @@ __device__ __forceinline__ void acc1(Real& A, Real& B, const TV<Real> t, const X
 // of four samples the template runs of four candidates are fetched together (indices
 // clamped into the padded row, so fetches for inactive candidates stay in bounds and are
 // simply not used), then the active candidates accumulate.
-template <class Real, bool ConstW, class XT>
-__device__ __forceinline__ void window_sums_seg(const XT* __restrict__ X, const Real* Tg,
-                                                int32_t d, int32_t q, int32_t R, Real* A, Real* B) {
+template <class Real, bool ConstW, class XS>
+__device__ __forceinline__ void window_sums_seg(XS X, const Real* Tg, int32_t d, int32_t q,
+                                                int32_t R, Real* A, Real* B) {
+    using XT = std::remove_cv_t<std::remove_reference_t<decltype(X[0])>>;
     const const_ptr<Real> T = as_const(Tg);
     const int32_t F = d / q, rem = d - F * q;
     const int32_t nseg = R + F;
@@ -68,7 +103,7 @@ __device__ __forceinline__ void window_sums_seg(const XT* __restrict__ X, const 
     const int32_t ilo = -kTmplPad, ihi = d + kTmplPad - 4;
     for (int32_t c = 0; c < nseg; ++c) {
         const int32_t rlo = max(0, c - F + 1), rhi = min(R - 1, c);
-        const XT* xs = X + (int64_t)c * q;
+        const XS xs = X + (int64_t)c * q;
         if (rlo <= rhi) {
             int32_t b = 0;
             // the next group of four samples is requested before the current one is used
@@ -124,9 +159,10 @@ __device__ __forceinline__ void window_sums_seg(const XT* __restrict__ X, const 
 
 // Stride 1 with 8 candidates per lane: sample j feeds candidates r with template index
 // j - r; eight samples at a time use the zero-padded template span [j0 - 7, j0 + 7].
-template <class Real, bool ConstW, class XT>
-__device__ __forceinline__ void window_sums_q1(const XT* __restrict__ X, const Real* Tg,
-                                               int32_t d, Real* A, Real* B) {
+template <class Real, bool ConstW, class XS>
+__device__ __forceinline__ void window_sums_q1(XS X, const Real* Tg, int32_t d, Real* A,
+                                               Real* B) {
+    using XT = std::remove_cv_t<std::remove_reference_t<decltype(X[0])>>;
     const const_ptr<Real> T = as_const(Tg);
     const int32_t L = d + kMaxBlock - 1;
     for (int32_t j0 = 0; j0 < L; j0 += 8) {
@@ -168,8 +204,8 @@ namespace gtls {
 // ---------------------------------------------------------------------------------------
 typedef float mfma_v4 __attribute__((ext_vector_type(4)));
 constexpr int kRingRows = 80;     // 64 rows of a super-tile + up to 15 look-ahead rows
-constexpr int kRingStride = 17;   // 16 chunk columns + 1 (bank spread)
-constexpr int kRingFloats = kRingRows * kRingStride;
+constexpr int kRingPitch = 84;    // column-major: column c at c * kRingPitch (16-byte rows of 4)
+constexpr int kRingFloats = 16 * kRingPitch;
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -190,56 +226,124 @@ __device__ __forceinline__ FV<M> ldm(const float* p) {
     return r;
 }
 
-// 2 M consecutive floats split into even (first) and odd (second) elements.
-template <int M>
-__device__ __forceinline__ void ldm2(const float* p, FV<M>& e, FV<M>& o) {
-    FV<2 * M> r;
-    __builtin_memcpy(&r, p, sizeof(r));
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        e.v[i] = r.v[2 * i];
-        o.v[i] = r.v[2 * i + 1];
-    }
-}
-
-template <int M>
-__device__ __forceinline__ void maskm(FV<M>& x, int32_t b, int32_t lim) {
-#pragma unroll
-    for (int i = 0; i < M; ++i) x.v[i] = (b + i < lim) ? x.v[i] : 0.f;
-}
-
 __device__ __forceinline__ int ring_wrap(int r) {
     r = r < 0 ? r + kRingRows : r;
     return r >= kRingRows ? r - kRingRows : r;
 }
 
-// MFMA steps of stride q with groups of 4 M samples.
-__host__ __device__ constexpr int32_t mfma_steps(int32_t q, int32_t m) {
-    return m * ((q + 4 * m - 1) / (4 * m));
+
+// K-group size of the matrix-core path: 4 M samples per group, M MFMA steps per tile. Wide
+// strides take long groups so that the MFMAs of one group cover the latency of the next
+// group's loads; short strides take short groups so that little of a group is padding.
+__host__ __device__ inline int32_t mfma_m(int32_t q) { return q >= 48 ? 8 : (q >= 16 ? 4 : 2); }
+
+// Chunked template rows of the matrix-core path: for width w with stride q, F = d / q and
+// q_pad = q rounded up to an even number of K groups (8 mfma_m(q) samples), chunk c <= F
+// sits at chunk_offset(w) + c q_pad, holds S[c q + b] for b < its length (q, or d - F q
+// for c = F) and zeros up to q_pad. Columns c > F read a block of zeros instead.
+// (F + 1) q_pad <= 2 d + 1008 for F <= 15, so the offsets below leave room for every row.
+__host__ __device__ inline int64_t chunk_offset(int64_t tmpl_off, int64_t w) {
+    return 2 * tmpl_off + 1024 * w;
+}
+__host__ __device__ inline int32_t chunk_pad(int32_t q) {
+    const int32_t g = 8 * mfma_m(q);
+    return (q + g - 1) / g * g;
 }
 
-// X: the item's gathered samples (u, or interleaved (v, w) when !ConstW); samples at or
-// past np are treated as 0 (read, never used unmasked; reads stay below
-// np + d + q + 16). T: the width's template (S, or interleaved (S, S^2)); reads past the
-// row (at most q + 15 samples) land in following rows or the zeroed tail of the template
-// buffer and are masked. D: prefix sums of u (float64) for the window means.
-// emit(k, A, B, Dlo, Dhi) is called by one lane per candidate k < K.
-template <bool ConstW, int M, class Emit>
-__device__ __forceinline__ void window_sums_mfma_m(const float* __restrict__ X, int64_t np,
-                                                   const float* __restrict__ T,
-                                                   const double* __restrict__ D, int32_t d,
-                                                   int32_t q, int64_t K, float* ringA,
-                                                   float* ringB, Emit emit) {
+// The 64 G rows of super-tile st for one (sample plane X, chunked template C) pair,
+// written to the ring. A operand X[row q + b]: samples b >= q meet zero template entries
+// and rows past the data read block 0, so the sample plane needs only finite values (the
+// gather zero-fills its tail); B operand C[c q_pad + b]. No masking in the loop: the loads
+// of the next group stay in flight while the MFMAs of the current one issue.
+template <int M>
+__device__ __forceinline__ void super_tile_rows(const float* __restrict__ X,
+                                                const float* __restrict__ C,
+                                                const float* __restrict__ Z, int32_t q,
+                                                int32_t F, int64_t rows, int64_t st, int ring0,
+                                                float* ring) {
     constexpr int GS = 4 * M;
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
-    const int32_t F = d / q, rem = d - F * q;
+    const float* tc = (col <= F) ? C + (int64_t)col * chunk_pad(q) : Z;
+    // an even number of groups: the last one of an odd count meets zero template entries
+    const int32_t ngroups = (((q + GS - 1) / GS) + 1) & ~1;
+    const float* xa_[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int64_t row = 64 * st + 16 * t + col;
+        xa_[t] = X + (row < rows ? row * q : 0) + M * kk;
+    }
+    tc += M * kk;
+    mfma_v4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+    // two explicit operand sets, loads always issued (the last group is fetched again at
+    // the end), so the loop body is straight-line and each MFMA block waits only for its
+    // own set while the other set's loads are in flight
+    FV<M> a0[4], b0, a1[4], b1;
+    auto load = [&](int32_t g, FV<M>(&xa)[4], FV<M>& xb) {
+        const int32_t o = GS * min(g, ngroups - 1);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) xa[t] = ldm<M>(xa_[t] + o);
+        xb = ldm<M>(tc + o);
+    };
+    auto mma = [&](const FV<M>(&xa)[4], const FV<M>& xb) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[t].v[i], xb.v[i], acc[t], 0, 0, 0);
+        }
+    };
+    // the scheduling barriers keep each load block ahead of the MFMA block before it, so
+    // the loads of one set overlap the MFMAs of the other
+    load(0, a0, b0);
+    for (int32_t g = 0; g < ngroups; g += 2) {
+        load(g + 1, a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        load(g + 2, a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // rows 16 t + 4 kk .. + 3 of column col: one 16-byte store (ring0 is a multiple of 16
+    // and kRingRows of 4, so the four rows never wrap apart)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        int r = ring0 + 16 * t + 4 * kk;
+        r = r >= kRingRows ? r - kRingRows : r;
+        *(mfma_v4*)(ring + col * kRingPitch + r) = acc[t];
+    }
+}
+
+// The window rings of the four wavefronts of a workgroup (A sums, then B sums).
+__shared__ __attribute__((aligned(16))) float g_rings[2 * 4 * kRingFloats];
+
+// Every candidate of one (period, width) item on the matrix cores, returning the lane's
+// first minimum of the statistic. X, C: the sample plane and chunked template of the A
+// sums (u with S when ConstW, v = u w with S otherwise); X2, C2: those of the B sums
+// (w with S^2; unused when ConstW); Z: zeros (at least q_pad floats). Reads of X stay
+// below N + d + q + 64 (zero-filled past the gathered samples). D: prefix sums of u
+// (float64) for the window means.
+template <bool ConstW, int M>
+__device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
+                                          const float* __restrict__ X2,
+                                          const float* __restrict__ C,
+                                          const float* __restrict__ C2,
+                                          const float* __restrict__ Z,
+                                          const double* __restrict__ D, WidthArgs a) {
+    const int lane = lane_id();
+    const int wave = uniform((int32_t)wave_id());
+    float* ringA = g_rings + wave * kRingFloats;
+    float* ringB = g_rings + (4 + wave) * kRingFloats;
+    const int32_t d = a.d, q = a.q;
+    const int64_t K = a.K;
+    const int32_t F = d / q;
     const int64_t rows = K + F;
     const int64_t nsuper = (rows + 63) / 64;
-    const int32_t clen = (col < F) ? q : (col == F ? rem : 0);   // valid b of this chunk
-    // columns past the last chunk read chunk 0 (masked)
-    const int64_t tcol = (col <= F) ? (int64_t)col * q : 0;
-    const int32_t ngroups = (q + GS - 1) / GS;
+    ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
     int ring0 = 0;   // ring row of G row 64 T
     for (int64_t st = 0; st < nsuper; ++st) {
         // the window-mean prefix sums of this super-tile's candidates, requested early
@@ -251,110 +355,53 @@ __device__ __forceinline__ void window_sums_mfma_m(const float* __restrict__ X, 
             dhi = D[s + d - 1];
             dlo = s ? D[s - 1] : 0.0;
         }
-        int64_t abase[4];
-        int32_t alim[4];
+        super_tile_rows<M>(X, C, Z, q, F, rows, st, ring0, ringA);
+        if constexpr (!ConstW) super_tile_rows<M>(X2, C2, Z, q, F, rows, st, ring0, ringB);
+        wave_sync_lds();
+        // diagonal sums: all 16 reads issue before the first add, columns past F masked
+        float va[16], vb[16];
+        const int r0 = ring0 - F + lane;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int64_t row = 64 * st + 16 * t + col;
-            const bool ok = row < rows;
-            // rows past the last read block 0 (masked)
-            abase[t] = ok ? row * q : 0;
-            alim[t] = ok ? (int32_t)min((int64_t)q, max((int64_t)0, np - row * q)) : 0;
+        for (int c = 0; c < 16; ++c) {
+            const int r = ring_wrap(r0 + min(c, F));
+            va[c] = ringA[c * kRingPitch + r];
+            if constexpr (!ConstW) vb[c] = ringB[c * kRingPitch + r];
         }
-        mfma_v4 acc[4], acc2[4];
+        float sa = 0.f, sb = 0.f;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
-            acc2[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
-        }
-        FV<M> a[4], a2[4], bv, bv2;
-        auto load_group = [&](int32_t g, FV<M>(&xa)[4], FV<M>(&xa2)[4], FV<M>& xb,
-                              FV<M>& xb2) {
-            const int32_t b = GS * g + M * kk;
-            if constexpr (ConstW) {
-                xb = ldm<M>(T + tcol + b);
-#pragma unroll
-                for (int t = 0; t < 4; ++t) xa[t] = ldm<M>(X + abase[t] + b);
-            } else {
-                ldm2<M>(T + 2 * (tcol + b), xb, xb2);
-#pragma unroll
-                for (int t = 0; t < 4; ++t) ldm2<M>(X + 2 * (abase[t] + b), xa[t], xa2[t]);
-            }
-            maskm<M>(xb, b, clen);
-            if constexpr (!ConstW) maskm<M>(xb2, b, clen);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                maskm<M>(xa[t], b, alim[t]);
-                if constexpr (!ConstW) maskm<M>(xa2[t], b, alim[t]);
-            }
-        };
-        load_group(0, a, a2, bv, bv2);
-        for (int32_t g = 0; g < ngroups; ++g) {
-            FV<M> na[4], na2[4], nb, nb2;
-            const bool more = g + 1 < ngroups;
-            if (more) load_group(g + 1, na, na2, nb, nb2);
-#pragma unroll
-            for (int i = 0; i < M; ++i) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t].v[i], bv.v[i], acc[t],
-                                                                  0, 0, 0);
-                    if constexpr (!ConstW)
-                        acc2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2[t].v[i], bv2.v[i],
-                                                                       acc2[t], 0, 0, 0);
-                }
-            }
-            if (more) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    a[t] = na[t];
-                    if constexpr (!ConstW) a2[t] = na2[t];
-                }
-                bv = nb;
-                if constexpr (!ConstW) bv2 = nb2;
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int r = ring0 + 16 * t + 4 * kk + v;
-                r = r >= kRingRows ? r - kRingRows : r;
-                ringA[r * kRingStride + col] = acc[t][v];
-                if constexpr (!ConstW) ringB[r * kRingStride + col] = acc2[t][v];
-            }
+        for (int c = 0; c < 16; ++c) {
+            sa += c <= F ? va[c] : 0.f;
+            if constexpr (!ConstW) sb += c <= F ? vb[c] : 0.f;
         }
         wave_sync_lds();
         if (cand) {
-            float sa = 0.f, sb = 0.f;
-            const int r0 = ring0 - F + lane;
-            for (int32_t c = 0; c <= F; ++c) {
-                const int r = ring_wrap(r0 + c);
-                sa += ringA[r * kRingStride + c];
-                if constexpr (!ConstW) sb += ringB[r * kRingStride + c];
+            const int32_t s = (int32_t)(kc * q);
+            const double m = (dhi - dlo) * a.inv_d;
+            if (m > a.depth_min) {
+                const float rs = (float)(m * a.os2);
+                const double delta =
+                    ConstW ? a.wc * ((double)(rs * rs) * a.ssk - 2.0 * (double)rs * (double)sa)
+                           : (double)(rs * rs * sb - 2.f * rs * sa);
+                consider(best, a.chi2f + delta, a.k, s);
+            } else {
+                consider(best, a.sentinel, a.k, s);
             }
-            emit(kc, sa, sb, dlo, dhi);
         }
-        wave_sync_lds();
         ring0 = ring0 + 64 >= kRingRows ? ring0 + 64 - kRingRows : ring0 + 64;
     }
+    return best;
 }
 
-// Picks the group size (4, 8 or 16 samples) with the fewest MFMA steps for stride q,
-// preferring larger groups (fewer, wider loads) on ties.
-template <bool ConstW, class Emit>
-__device__ __forceinline__ void window_sums_mfma(const float* __restrict__ X, int64_t np,
-                                                 const float* __restrict__ T,
-                                                 const double* __restrict__ D, int32_t d,
-                                                 int32_t q, int64_t K, float* ringA,
-                                                 float* ringB, Emit emit) {
-    const int32_t s1 = mfma_steps(q, 1), s2 = mfma_steps(q, 2), s4 = mfma_steps(q, 4);
-    if (s4 <= s1 && s4 <= s2)
-        window_sums_mfma_m<ConstW, 4>(X, np, T, D, d, q, K, ringA, ringB, emit);
-    else if (s2 <= s1)
-        window_sums_mfma_m<ConstW, 2>(X, np, T, D, d, q, K, ringA, ringB, emit);
-    else
-        window_sums_mfma_m<ConstW, 1>(X, np, T, D, d, q, K, ringA, ringB, emit);
+// Dispatches on the weight mode and the K-group size of stride q (mfma_m).
+template <bool ConstW>
+__device__ __forceinline__ ArgMin window_sums_mfma(const float* X, const float* X2,
+                                                   const float* C, const float* C2,
+                                                   const float* Z, const double* D,
+                                                   const WidthArgs& a) {
+    const int32_t m = mfma_m(a.q);
+    if (m == 8) return mfma_width<ConstW, 8>(X, X2, C, C2, Z, D, a);
+    if (m == 4) return mfma_width<ConstW, 4>(X, X2, C, C2, Z, D, a);
+    return mfma_width<ConstW, 2>(X, X2, C, C2, Z, D, a);
 }
 
 }  // namespace gtls
