@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _ffi
 from . import constants as C
-from .search import MODES, _ctx, _device, _ptr, make_plan
+from .search import MODES, _ctx, _device, _ptr, make_plan, workspace
 from .validate import validate_args, validate_inputs
 
 
@@ -42,9 +42,10 @@ class BatchPlan:
     def __init__(self, plans):
         self.plans = plans
         n = len(plans)
-        work = np.array([p.n * len(p.periods) * max(1, int(np.mean(p.k_hi - p.k_lo + 1)))
-                         for p in plans], dtype=np.float64)
-        self.order = np.argsort(-work, kind="stable")      # packed position -> curve
+        # longest curves first: the work queue hands out the largest items early, and
+        # consecutive curves have similar lengths (size groups, DeviceBatch)
+        npts = np.array([p.n for p in plans], dtype=np.int64)
+        self.order = np.argsort(-npts, kind="stable")      # packed position -> curve
         ps = [plans[i] for i in self.order]
         self.n_curves = n
         self.pt_off = np.zeros(n + 1, np.int64)
@@ -91,8 +92,22 @@ class BatchPlan:
         return s
 
 
+def size_groups(bp, ratio=2.0):
+    """Contiguous ranges [c0, c1) of packed curves (longest first) in which no curve is
+    more than `ratio` times shorter than the first: each range is searched with its own
+    point capacity, so short curves get chunks of many more periods."""
+    n = np.diff(bp.pt_off)
+    groups, c0 = [], 0
+    for c in range(1, len(n) + 1):
+        if c == len(n) or n[c] * ratio < n[c0]:
+            groups.append((c0, c))
+            c0 = c
+    return groups
+
+
 class DeviceBatch:
-    """Device-resident BatchPlan; search() enqueues one persistent search launch."""
+    """Device-resident BatchPlan; search() enqueues one persistent search launch per size
+    group of curves (size_groups)."""
 
     def __init__(self, bp: BatchPlan, device=0):
         import torch
@@ -118,25 +133,52 @@ class DeviceBatch:
         self.chi2 = torch.empty(npd, dtype=torch.float64, device=dev)
         self.best_k = torch.empty(npd, dtype=torch.int32, device=dev)
         self.best_s = torch.empty(npd, dtype=torch.int32, device=dev)
-        self._ws = None
+        # per size group: offsets re-based to the group's first curve
+        self.groups = []
+        for c0, c1 in size_groups(bp):
+            p0, p1 = int(bp.per_off[c0]), int(bp.per_off[c1])
+            w0, w1 = int(bp.wid_off[c0]), int(bp.wid_off[c1])
+            t0 = int(bp.pt_off[c0])
+            tm0 = int(bp.tmpl_off[w0]) if w1 > w0 else 0
+            tm1 = int(bp.tmpl_off[w1]) if w1 < len(bp.tmpl_off) else len(bp.tmpl)
+            ok = bp.k_lo[p0:p1] <= bp.k_hi[p0:p1]
+            kw = (bp.k_hi[p0:p1] - bp.k_lo[p0:p1] + 1)[ok]
+            self.groups.append(dict(
+                c0=c0, c1=c1, p0=p0, p1=p1, w0=w0, t0=t0, tm0=tm0,
+                pt_off=i64(bp.pt_off[c0:c1 + 1] - t0), per_off=i64(bp.per_off[c0:c1 + 1] - p0),
+                wid_off=i64(bp.wid_off[c0:c1 + 1] - w0),
+                tmpl_off=i64(bp.tmpl_off[w0:w1] - tm0),
+                total_points=int(bp.pt_off[c1]) - t0, total_periods=p1 - p0,
+                total_widths=w1 - w0, total_tmpl=tm1 - tm0,
+                max_points=int(np.max(np.diff(bp.pt_off[c0:c1 + 1]))),
+                max_width=int(bp.widths[w0:w1].max()),
+                max_period_widths=int(max(1, kw.max(initial=1)))))
 
-    def batch_struct(self, skip_point, depth_min):
+    def batch_struct(self, skip_point, depth_min, g):
+        """The C batch of size group g: re-based offsets, data pointers into the batch's
+        arrays at the group's first curve / period / width / template sample."""
         a, bp = self.arrays, self.bp
         b = _ffi.GtlsBatch()
-        b.n_curves = bp.n_curves
-        for name in ("pt_off", "t", "y", "dy", "per_off", "periods", "k_lo", "k_hi",
-                     "wid_off", "widths", "overshoot", "tmpl_off", "tmpl"):
-            setattr(b, name, a[name].data_ptr())
+        b.n_curves = g["c1"] - g["c0"]
+        b.pt_off = g["pt_off"].data_ptr()
+        b.per_off = g["per_off"].data_ptr()
+        b.wid_off = g["wid_off"].data_ptr()
+        b.tmpl_off = g["tmpl_off"].data_ptr()
+        for name, base in (("t", g["t0"]), ("y", g["t0"]), ("dy", g["t0"]),
+                           ("periods", g["p0"]), ("k_lo", g["p0"]), ("k_hi", g["p0"]),
+                           ("widths", g["w0"]), ("overshoot", g["w0"]), ("tmpl", g["tm0"])):
+            x = a[name]
+            setattr(b, name, x.data_ptr() + base * x.element_size())
         b.skip_point = int(skip_point)
         b.depth_min = float(depth_min)
         b.negative_time = bp.negative_time
-        b.total_points = int(bp.pt_off[-1])
-        b.total_periods = bp.total_periods
-        b.total_widths = int(bp.wid_off[-1])
-        b.total_tmpl = int(len(bp.tmpl))
-        b.max_points = bp.max_points
-        b.max_width = bp.max_width
-        b.max_period_widths = bp.max_period_widths
+        b.total_points = g["total_points"]
+        b.total_periods = g["total_periods"]
+        b.total_widths = g["total_widths"]
+        b.total_tmpl = g["total_tmpl"]
+        b.max_points = g["max_points"]
+        b.max_width = g["max_width"]
+        b.max_period_widths = g["max_period_widths"]
         return b
 
     def spectra(self, chi2, oversampling_factor=C.OVERSAMPLING_FACTOR, sync=True):
@@ -185,19 +227,26 @@ class DeviceBatch:
                     device=status == 0)
 
     def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
-        """Enqueue the batched search on torch's current stream (no synchronisation)."""
-        import torch
+        """Enqueue the batched search on torch's current stream (no synchronisation): one
+        gtls_search_batched call per size group, each writing its slice of the outputs."""
         m = MODES[mode]
         ctx = _ctx(self.dev)
-        b = self.batch_struct(skip_point, depth_min)
         lib = _ffi.load()
-        need = lib.gtls_search_workspace_bytes(ctx.handle, m, ct.byref(b))
-        if need < 0:
-            _ffi.check(ctx.handle, int(need))
-        if self._ws is None or self._ws.numel() < need:
-            self._ws = torch.empty(int(need), dtype=torch.uint8, device=self.dev)
-        code = lib.gtls_search_batched(ctx.handle, m, ct.byref(b), _ptr(self._ws),
-                                       self._ws.numel(), _ptr(self.chi2), _ptr(self.best_k),
-                                       _ptr(self.best_s))
-        _ffi.check(ctx.handle, code)
+        structs = [self.batch_struct(skip_point, depth_min, g) for g in self.groups]
+        need = 0
+        for b in structs:
+            nb = lib.gtls_search_workspace_bytes(ctx.handle, m, ct.byref(b))
+            if nb < 0:
+                _ffi.check(ctx.handle, int(nb))
+            need = max(need, int(nb))
+        ws = workspace(self.dev, need)
+        for g, b in zip(self.groups, structs):
+            if b.total_periods == 0:
+                continue
+            code = lib.gtls_search_batched(
+                ctx.handle, m, ct.byref(b), _ptr(ws), ws.numel(),
+                ct.c_void_p(self.chi2.data_ptr() + 8 * g["p0"]),
+                ct.c_void_p(self.best_k.data_ptr() + 4 * g["p0"]),
+                ct.c_void_p(self.best_s.data_ptr() + 4 * g["p0"]))
+            _ffi.check(ctx.handle, code)
         return self.chi2, self.best_k, self.best_s

@@ -16,6 +16,12 @@
 // samples) are sorted by one wavefront with a bitonic network in registers, larger ones
 // by the workgroup in LDS, and the largest (e.g. repeated times) by LDS-sorted chunks and
 // rank-based merges. The cost of a bucket is bounded by m log^2 m in every case.
+//
+// Windowed form (wg_fold_sort_windows, the usual path): after the histogram, the buckets
+// are cut into windows of about kWinTarget samples; for each window the workgroup folds t
+// again, stages the window's samples in LDS at their bucket positions, orders every bucket
+// there and hands the window to a sink, so nothing is scattered through global memory.
+// A period whose largest bucket exceeds kWaveSortMax keys takes the global form above.
 #pragma once
 #include "wg_primitives.h"
 
@@ -376,6 +382,163 @@ __device__ __forceinline__ void wg_fold_sort(const double* __restrict__ t, int64
         else block_merge_sort(key_out, perm, tmp_key, tmp_id, lo, cnt, sk, si);
     }
     __syncthreads();
+}
+
+constexpr int kWinCap = 3072;                        // samples one window stages in LDS
+constexpr int kWinTarget = kWinCap - kWaveSortMax;   // window size before its last bucket
+constexpr int kWinMid = kWinCap / 17 + 1;            // buckets of 17+ keys in one window
+constexpr int kCompU = 8;                            // bucket indices per lane per step
+constexpr int kCompPerWave = 64 * kCompU;            // packed in-window samples per step
+
+// Windowed stable phase sort of the n samples of t for period P (see the file comment).
+// Scratch: bk (n uint16: each sample's bucket, written by the histogram pass and read by
+// the window passes, which fold again only the samples of their window: each wavefront
+// packs the indices of its in-window samples into comp (its kCompPerWave ints of LDS) and
+// folds them one per lane).
+// LDS: hist (kMaxBuckets ints), red (blockDim/64 ints), counts (4 ints), sphi / sidx
+// (kWinCap doubles / ints), mids (kWinMid int64), comp (kCompPerWave ints per wavefront).
+// sink(wstart, wcnt) is called by every
+// thread for each window: the samples of sorted positions wstart .. wstart + wcnt - 1 sit
+// in sphi / sidx[0 .. wcnt); it may use barriers. Returns false (every thread) when a
+// bucket holds more than kWaveSortMax keys: nothing was sorted, the caller takes the
+// global form.
+template <class Sink>
+__device__ __forceinline__ bool wg_fold_sort_windows(const double* __restrict__ t, int64_t n,
+                                                     double P, bool neg,
+                                                     uint16_t* __restrict__ bk, int* hist,
+                                                     int* red, int* counts, double* sphi,
+                                                     int32_t* sidx, int64_t* mids, int* comp,
+                                                     Sink sink) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int B = buckets_for(n);
+    for (int b = tid; b < B; b += nt) hist[b] = 0;
+    if (tid < 4) counts[tid] = 0;
+    __syncthreads();
+    constexpr int U = 4;   // samples in flight per thread
+    for (int64_t base = 0; base < n; base += (int64_t)U * nt) {
+        double tv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = base + (int64_t)u * nt + tid;
+            tv[u] = j < n ? t[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = base + (int64_t)u * nt + tid;
+            if (j < n) {
+                const int b = phase_bucket(fold_phase(tv[u], P), B, neg);
+                atomicAdd(&hist[b], 1);
+                bk[j] = (uint16_t)b;
+            }
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the histogram; flag a bucket too large for the windowed form
+    {
+        const int per = B / nt;  // B is a multiple of 256 >= blockDim
+        const int base = tid * per;
+        int sum = 0, mx = 0;
+        for (int e = 0; e < per; ++e) {
+            const int v = hist[base + e];
+            sum += v;
+            mx = max(mx, v);
+        }
+        int total;
+        int run = block_exclusive_scan(sum, red, total);
+        for (int e = 0; e < per; ++e) {
+            const int v = hist[base + e];
+            hist[base + e] = run;
+            run += v;
+        }
+        if (mx > kWaveSortMax) atomicOr(&counts[0], 1);
+    }
+    __syncthreads();
+    if (counts[0] != 0) return false;
+    int bfirst = 0;
+    int wstart = 0;
+    while (bfirst < B) {
+        // the window: buckets bfirst .. bnext - 1, bnext the first bucket starting at or past
+        // wstart + kWinTarget (so wcnt < kWinTarget + kWaveSortMax = kWinCap)
+        int lo = bfirst + 1, hi = B;
+        const int target = wstart + kWinTarget;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (hist[mid] >= target) hi = mid; else lo = mid + 1;
+        }
+        const int bnext = lo;
+        const int wend = bnext < B ? hist[bnext] : (int)n;
+        const int wcnt = wend - wstart;
+        if (tid == 0) counts[1] = 0;
+        __syncthreads();   // every thread has read hist before the pass moves the cursors
+        {
+            // each wavefront walks its own slices of kCompPerWave samples
+            const int lane = lane_id();
+            const int wave = uniform((int32_t)wave_id());
+            const int nwaves = nt / kWave;
+            int* wc = comp + wave * kCompPerWave;
+            for (int64_t base = (int64_t)wave * kCompPerWave; base < n;
+                 base += (int64_t)nwaves * kCompPerWave) {
+                int bv[kCompU];
+                int cnt = 0;
+#pragma unroll
+                for (int u = 0; u < kCompU; ++u) {
+                    const int64_t j = base + u * kWave + lane;
+                    bv[u] = j < n ? (int)bk[j] : -1;
+                    cnt += (bv[u] >= bfirst && bv[u] < bnext) ? 1 : 0;
+                }
+                const int incl = wave_inclusive_scan(cnt);
+                const int total = __shfl(incl, kWave - 1, kWave);
+                int o = incl - cnt;
+#pragma unroll
+                for (int u = 0; u < kCompU; ++u) {
+                    if (bv[u] >= bfirst && bv[u] < bnext) wc[o++] = (int)(base + u * kWave + lane);
+                }
+                wave_sync_lds();
+                for (int e = lane; e < total; e += kWave) {
+                    const int j = wc[e];
+                    const double ph = fold_phase(t[j], P);
+                    const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1) - wstart;
+                    sphi[pos] = ph;
+                    sidx[pos] = (int32_t)j;
+                }
+                wave_sync_lds();
+            }
+        }
+        __syncthreads();
+        // hist[b] now holds the end of bucket b for the window's buckets
+        for (int b = bfirst + tid; b < bnext; b += nt) {
+            const int blo = (b > bfirst ? hist[b - 1] : wstart) - wstart;
+            const int cnt = hist[b] - wstart - blo;
+            if (cnt <= 1) continue;
+            if (cnt <= 8) {
+                sort_bucket_regs<8>(sphi, sidx, blo, cnt);
+            } else if (cnt <= 16) {
+                sort_bucket_regs<16>(sphi, sidx, blo, cnt);
+            } else {
+                const int e = atomicAdd(&counts[1], 1);
+                mids[e] = ((int64_t)blo << 32) | (int64_t)cnt;
+            }
+        }
+        __syncthreads();
+        {
+            const int nmid = counts[1];
+            const int wave = uniform((int32_t)wave_id());
+            for (int e = wave; e < nmid; e += nt / kWave) {
+                const int64_t ent = uniform(mids[e]);
+                const int64_t blo = ent >> 32;
+                const int cnt = (int)(ent & 0xffffffffLL);
+                if (cnt <= 64) wave_bitonic<1>(sphi, sidx, blo, cnt);
+                else if (cnt <= 128) wave_bitonic<2>(sphi, sidx, blo, cnt);
+                else wave_bitonic<4>(sphi, sidx, blo, cnt);
+            }
+        }
+        __syncthreads();
+        sink(wstart, wcnt);
+        __syncthreads();
+        bfirst = bnext;
+        wstart = wend;
+    }
+    return true;
 }
 
 }  // namespace gtls

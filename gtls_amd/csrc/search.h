@@ -72,7 +72,7 @@ struct SearchParams {
     // prep_kernel's per-workgroup sort scratch
     char* slots;
     int64_t slot_bytes;
-    int64_t o_key, o_perm, o_tmp_key, o_tmp_id, o_lists;
+    int64_t o_key, o_perm, o_tmp_key, o_tmp_id, o_lists, o_bk;
     // per chunk period i: gathered samples in xbuf[i * xstride ...] (room for 2 xstride
     // Reals: u, or the planes v = u w at [0, xstride) and w at [xstride, 2 xstride)) and
     // their float64 running sum dbuf[i * xstride ...]
@@ -105,6 +105,58 @@ __device__ __forceinline__ int xcd_id() {
     return v & 7;
 }
 
+// Gathers the samples of sorted positions o0 .. o0 + cnt - 1 (source sample src(j) for
+// j < cnt) in the search precision -- u alone with a common weight, else the planes
+// v = u w and w -- and writes the float64 running sum D of u, continuing from carry
+// (updated; equal in every thread). All threads call it; red_d is LDS of blockDim/64.
+template <class Real, class Src>
+__device__ __forceinline__ void gather_scan(const typename Pair<Real>::type* __restrict__ uw,
+                                            int64_t o0, int64_t cnt, Src src, bool constw,
+                                            Real* ub, Real* wpl, double* D, double& carry,
+                                            double* red_d) {
+    using R2 = typename Pair<Real>::type;
+    const int lane = lane_id(), wave = wave_id();
+    const int64_t chunk = (int64_t)kBlock * kScanPerThread;
+    for (int64_t base = 0; base < cnt; base += chunk) {
+        const int64_t wbase = base + (int64_t)wave * kWave * kScanPerThread;
+        double u[kScanPerThread];
+        double run = 0.0;
+#pragma unroll
+        for (int e = 0; e < kScanPerThread; ++e) {
+            const int64_t j = wbase + (int64_t)e * kWave + lane;
+            Real uu = 0, ww = 0;
+            if (j < cnt) {
+                const R2 a = uw[src(j)];
+                uu = a.x;
+                ww = a.y;
+                if (constw) {
+                    ub[o0 + j] = uu;
+                } else {
+                    ub[o0 + j] = uu * ww;
+                    wpl[o0 + j] = ww;
+                }
+            }
+            const double sc = wave_inclusive_scan((double)uu);
+            u[e] = sc + run;
+            run += __shfl(sc, kWave - 1, kWave);
+        }
+        if (lane == kWave - 1) red_d[wave] = run;
+        __syncthreads();
+        double before = carry, total = carry;
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            if (w < wave) before += red_d[w];
+            total += red_d[w];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < kScanPerThread; ++e) {
+            const int64_t j = wbase + (int64_t)e * kWave + lane;
+            if (j < cnt) D[o0 + j] = u[e] + before;
+        }
+        carry = total;
+    }
+}
+
 // Stage 1 + 2 of each period of the chunk, one workgroup per period (persistent):
 //   1. fold + stable phase sort (fold_sort.h)                  -> perm
 //   2. gather in phase order with the wrap-around extension    -> X, running sum D
@@ -112,8 +164,12 @@ template <class Real>
 __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
     using R2 = typename Pair<Real>::type;
     __shared__ __attribute__((aligned(16))) int hist[kMaxBuckets];
+    __shared__ __attribute__((aligned(16))) double s_phi[kWinCap];
+    __shared__ int32_t s_idx[kWinCap];
+    __shared__ int64_t s_mids[kWinMid];
+    __shared__ int s_comp[(kBlock / kWave) * kCompPerWave];
     __shared__ int red_i[kBlock / kWave];
-    __shared__ int s_counts[2];
+    __shared__ int s_counts[4];
     __shared__ double red_d[kBlock / kWave];
     __shared__ unsigned long long s_item;
 
@@ -124,6 +180,7 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
     double* tmp_key = (double*)(slot + p.o_tmp_key);
     int32_t* tmp_id = (int32_t*)(slot + p.o_tmp_id);
     int64_t* lists = (int64_t*)(slot + p.o_lists);
+    uint16_t* bk = (uint16_t*)(slot + p.o_bk);
 
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&p.counters[0], 1ULL);
@@ -156,69 +213,47 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
         if (tid == 0) p.meta[i] = PeriodMeta{c, status, np_};
         if (status == 0) {
             const double P = p.periods[g];
-            wg_fold_sort(p.t + pt0, n, P, p.negative_time != 0, key, perm, tmp_key, tmp_id,
-                         lists, hist, red_i, s_counts);
-
             // gather in phase order (wrapping) and the float64 running sum of u. With a
             // common weight only u is kept (X = u), otherwise the planes v = u w and w.
             const double wc = p.wconst[c];
             const bool constw = wc == wc;
             Real* ub = (Real*)(p.xbuf + i * p.xstride);
-            Real* vpl = ub;
             Real* wpl = ub + p.xstride;
             double* D = p.dbuf + i * p.xstride;
-            const int lane = lane_id(), wave = wave_id();
-            const int64_t chunk = (int64_t)kBlock * kScanPerThread;
+            const R2* uw = p.uw0 + pt0;
             double carry = 0.0;
-            for (int64_t base = 0; base < np_; base += chunk) {
-                const int64_t wbase = base + (int64_t)wave * kWave * kScanPerThread;
-                double u[kScanPerThread];
-                double run = 0.0;
-#pragma unroll
-                for (int e = 0; e < kScanPerThread; ++e) {
-                    const int64_t j = wbase + (int64_t)e * kWave + lane;
-                    Real uu = 0, ww = 0;
-                    if (j < np_) {
-                        const int64_t jj = j < n ? j : j % n;
-                        const R2 a = p.uw0[pt0 + perm[jj]];
-                        uu = a.x;
-                        ww = a.y;
-                        if (constw) {
-                            ub[j] = uu;
-                        } else {
-                            vpl[j] = uu * ww;
-                            wpl[j] = ww;
-                        }
-                    }
-                    const double s = wave_inclusive_scan((double)uu);
-                    u[e] = s + run;
-                    run += __shfl(s, kWave - 1, kWave);
+            const bool windowed = wg_fold_sort_windows(
+                p.t + pt0, n, P, p.negative_time != 0, bk, hist, red_i, s_counts, s_phi, s_idx,
+                s_mids, s_comp, [&](int wstart, int wcnt) {
+                    gather_scan<Real>(uw, wstart, wcnt, [&](int64_t j) { return s_idx[j]; },
+                                      constw, ub, wpl, D, carry, red_d);
+                });
+            if (!windowed) {
+                // a bucket too large for LDS windows: the global form, then one gather pass
+                wg_fold_sort(p.t + pt0, n, P, p.negative_time != 0, key, perm, tmp_key, tmp_id,
+                             lists, hist, red_i, s_counts);
+                gather_scan<Real>(uw, 0, n, [&](int64_t j) { return perm[j]; }, constw, ub, wpl,
+                                  D, carry, red_d);
+            }
+            __syncthreads();
+            // the wrap-around extension: positions n .. np - 1 repeat positions 0 .. dmax - 1
+            for (int64_t j = n + tid; j < np_; j += kBlock) {
+                const int64_t jj = (j - n) % n;
+                const int64_t laps = (j - n) / n + 1;
+                if (constw) {
+                    ub[j] = ub[jj];
+                } else {
+                    ub[j] = ub[jj];
+                    wpl[j] = wpl[jj];
                 }
-                if (lane == kWave - 1) red_d[wave] = run;
-                __syncthreads();
-                double before = carry, total = carry;
-                for (int w = 0; w < kBlock / kWave; ++w) {
-                    if (w < wave) before += red_d[w];
-                    total += red_d[w];
-                }
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < kScanPerThread; ++e) {
-                    const int64_t j = wbase + (int64_t)e * kWave + lane;
-                    if (j < np_) D[j] = u[e] + before;
-                }
-                carry = total;
+                D[j] = (double)laps * carry + D[jj];
             }
             // zeros past the sequence, up to where the window loops read (below
             // N + d + q + 64 <= N + 2 dmax + 64): those samples meet zero template entries
             const int64_t zend = min(p.xstride, n + 2 * (int64_t)dmax + 96);
             for (int64_t j = np_ + tid; j < zend; j += kBlock) {
-                if (constw) {
-                    ub[j] = (Real)0;
-                } else {
-                    vpl[j] = (Real)0;
-                    wpl[j] = (Real)0;
-                }
+                ub[j] = (Real)0;
+                if (!constw) wpl[j] = (Real)0;
             }
         }
         __syncthreads();
@@ -399,9 +434,11 @@ __global__ void reduce_kernel(SearchParams<Real> p) {
 }
 
 // Stable phase permutations only (bit-exact checks of the sort). Slot layout: phases
-// (8 n), merge scratch (8 n + 4 n), bucket lists (8 kMaxBuckets).
+// (8 n), merge scratch (8 n + 4 n), bucket lists (8 kMaxBuckets), bucket of each sample
+// (2 n).
 __host__ __device__ inline int64_t fold_sort_slot_bytes(int64_t n) {
-    return ((8 * n + 255) / 256 + (12 * n + 255) / 256 + (8 * (int64_t)kMaxBuckets + 255) / 256) * 256;
+    return ((8 * n + 255) / 256 + (12 * n + 255) / 256 + (8 * (int64_t)kMaxBuckets + 255) / 256 +
+            (2 * n + 255) / 256) * 256;
 }
 
 __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int64_t n,
@@ -409,16 +446,30 @@ __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int6
                                                           int64_t n_periods, int neg,
                                                           char* slots, int32_t* perm_out) {
     __shared__ __attribute__((aligned(16))) int hist[kMaxBuckets];
+    __shared__ __attribute__((aligned(16))) double s_phi[kWinCap];
+    __shared__ int32_t s_idx[kWinCap];
+    __shared__ int64_t s_mids[kWinMid];
+    __shared__ int s_comp[(kBlock / kWave) * kCompPerWave];
     __shared__ int red_i[kBlock / kWave];
-    __shared__ int s_counts[2];
+    __shared__ int s_counts[4];
     char* slot = slots + (int64_t)blockIdx.x * fold_sort_slot_bytes(n);
     double* key = (double*)slot;
     double* tmp_key = (double*)(slot + (8 * n + 255) / 256 * 256);
     int32_t* tmp_id = (int32_t*)(tmp_key + n);
     int64_t* lists = (int64_t*)(slot + (8 * n + 255) / 256 * 256 + (12 * n + 255) / 256 * 256);
-    for (int64_t pi = blockIdx.x; pi < n_periods; pi += gridDim.x)
-        wg_fold_sort(t, n, periods[pi], neg != 0, key, perm_out + pi * n, tmp_key, tmp_id, lists,
-                     hist, red_i, s_counts);
+    uint16_t* bk = (uint16_t*)((char*)lists + (8 * (int64_t)kMaxBuckets + 255) / 256 * 256);
+    for (int64_t pi = blockIdx.x; pi < n_periods; pi += gridDim.x) {
+        int32_t* out = perm_out + pi * n;
+        const bool windowed = wg_fold_sort_windows(
+            t, n, periods[pi], neg != 0, bk, hist, red_i, s_counts, s_phi, s_idx, s_mids, s_comp,
+            [&](int wstart, int wcnt) {
+                for (int j = threadIdx.x; j < wcnt; j += blockDim.x) out[wstart + j] = s_idx[j];
+            });
+        if (!windowed)
+            wg_fold_sort(t, n, periods[pi], neg != 0, key, out, tmp_key, tmp_id, lists, hist,
+                         red_i, s_counts);
+        __syncthreads();
+    }
 }
 
 template <class Real>

@@ -64,6 +64,13 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T* tmp, T& total) {
 }
 
 // Lexicographic (value, k, s) minimum: the first minimum in (k, s) order wins ties.
+// Orders the LDS accesses of one wavefront before and after this point.
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct ArgMin {
     double v;
     int32_t k;

@@ -207,11 +207,6 @@ constexpr int kRingRows = 80;     // 64 rows of a super-tile + up to 15 look-ahe
 constexpr int kRingPitch = 84;    // column-major: column c at c * kRingPitch (16-byte rows of 4)
 constexpr int kRingFloats = 16 * kRingPitch;
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int M>
 struct FV {

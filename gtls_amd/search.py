@@ -86,6 +86,22 @@ def _ptr(x):
     return ct.c_void_p(x.data_ptr())
 
 
+_WORKSPACES = {}
+
+
+def workspace(dev, nbytes):
+    """Search workspace shared by every search on (device, current stream): searches on
+    one stream run in order, so they can reuse one buffer; it grows to the largest need."""
+    torch = _torch()
+    key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
+    ws = _WORKSPACES.get(key)
+    if ws is None or ws.numel() < nbytes:
+        _WORKSPACES.pop(key, None)
+        ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        _WORKSPACES[key] = ws
+    return ws
+
+
 class DeviceCurve:
     """Device-resident arrays of one curve (reused across repeated searches)."""
 
@@ -108,7 +124,6 @@ class DeviceCurve:
         self.n_widths = tb.n_widths
         self.max_width = int(tb.widths.max())
         self.negative_time = int(np.min(plan.t) < 0)
-        self._ws = None
 
     def workspace(self, ctx, mode, n_periods):
         lib = _ffi.load()
@@ -117,9 +132,7 @@ class DeviceCurve:
                                                  self.max_width)
         if need < 0:
             _ffi.check(ctx.handle, int(need))
-        if self._ws is None or self._ws.numel() < need:
-            self._ws = _torch().empty(int(need), dtype=_torch().uint8, device=self.dev)
-        return self._ws
+        return workspace(self.dev, need)
 
     def search(self, periods, k_lo, k_hi, mode="fp64", skip_point=8,
                depth_min=C.TRANSIT_DEPTH_MIN, sync=True):
