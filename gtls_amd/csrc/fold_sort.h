@@ -18,10 +18,11 @@
 // rank-based merges. The cost of a bucket is bounded by m log^2 m in every case.
 //
 // Windowed form (wg_fold_sort_windows, the usual path): after the histogram, the buckets
-// are cut into windows of about kWinTarget samples; for each window the workgroup folds t
-// again, stages the window's samples in LDS at their bucket positions, orders every bucket
-// there and hands the window to a sink, so nothing is scattered through global memory.
-// A period whose largest bucket exceeds kWaveSortMax keys takes the global form above.
+// are cut into windows of about kWinTarget samples and one pass sends every sample index
+// to its window's range (a few dozen ranges, so partly written lines stay in cache until
+// they fill); for each window the workgroup folds its samples again, stages them in LDS at
+// their bucket positions, orders every bucket there and hands the window to a sink. A
+// period whose largest bucket exceeds kWaveSortMax keys takes the global form above.
 #pragma once
 #include "wg_primitives.h"
 
@@ -384,31 +385,27 @@ __device__ __forceinline__ void wg_fold_sort(const double* __restrict__ t, int64
     __syncthreads();
 }
 
-constexpr int kWinCap = 3072;                        // samples one window stages in LDS
+constexpr int kWinCap = 2816;                        // samples one window stages in LDS
 constexpr int kWinTarget = kWinCap - kWaveSortMax;   // window size before its last bucket
 constexpr int kWinMid = kWinCap / 17 + 1;            // buckets of 17+ keys in one window
-constexpr int kCompU = 8;                            // bucket indices per lane per step
-constexpr int kCompPerWave = 64 * kCompU;            // packed in-window samples per step
+constexpr int kMaxWin = 255;                         // windows of one period (uint8 table)
 
 // Windowed stable phase sort of the n samples of t for period P (see the file comment).
-// Scratch: bk (n uint16: each sample's bucket, written by the histogram pass and read by
-// the window passes, which fold again only the samples of their window: each wavefront
-// packs the indices of its in-window samples into comp (its kCompPerWave ints of LDS) and
-// folds them one per lane).
-// LDS: hist (kMaxBuckets ints), red (blockDim/64 ints), counts (4 ints), sphi / sidx
-// (kWinCap doubles / ints), mids (kWinMid int64), comp (kCompPerWave ints per wavefront).
-// sink(wstart, wcnt) is called by every
-// thread for each window: the samples of sorted positions wstart .. wstart + wcnt - 1 sit
-// in sphi / sidx[0 .. wcnt); it may use barriers. Returns false (every thread) when a
-// bucket holds more than kWaveSortMax keys: nothing was sorted, the caller takes the
+// Scratch: bk (n uint16: each sample's bucket), binned (n ints: sample indices grouped by
+// window). LDS: hist (kMaxBuckets ints), red (blockDim/64 ints), counts (4 ints), sphi /
+// sidx (kWinCap doubles / ints), mids (kWinMid int64), wtab (kMaxBuckets bytes: window of
+// each bucket), wb / ws (kMaxWin + 1 ints: first bucket / first position of each window),
+// wcur (kMaxWin ints).
+// sink(wstart, wcnt) is called by every thread for each window: the samples of sorted
+// positions wstart .. wstart + wcnt - 1 sit in sphi / sidx[0 .. wcnt); it may use
+// barriers. Returns false (every thread) when a bucket holds more than kWaveSortMax keys
+// or the period needs more than kMaxWin windows: nothing was sorted, the caller takes the
 // global form.
 template <class Sink>
-__device__ __forceinline__ bool wg_fold_sort_windows(const double* __restrict__ t, int64_t n,
-                                                     double P, bool neg,
-                                                     uint16_t* __restrict__ bk, int* hist,
-                                                     int* red, int* counts, double* sphi,
-                                                     int32_t* sidx, int64_t* mids, int* comp,
-                                                     Sink sink) {
+__device__ __forceinline__ bool wg_fold_sort_windows(
+    const double* __restrict__ t, int64_t n, double P, bool neg, uint16_t* __restrict__ bk,
+    int32_t* __restrict__ binned, int* hist, int* red, int* counts, double* sphi,
+    int32_t* sidx, int64_t* mids, uint8_t* wtab, int* wb, int* ws, int* wcur, Sink sink) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int B = buckets_for(n);
     for (int b = tid; b < B; b += nt) hist[b] = 0;
@@ -454,55 +451,71 @@ __device__ __forceinline__ bool wg_fold_sort_windows(const double* __restrict__ 
     }
     __syncthreads();
     if (counts[0] != 0) return false;
-    int bfirst = 0;
-    int wstart = 0;
-    while (bfirst < B) {
-        // the window: buckets bfirst .. bnext - 1, bnext the first bucket starting at or past
-        // wstart + kWinTarget (so wcnt < kWinTarget + kWaveSortMax = kWinCap)
-        int lo = bfirst + 1, hi = B;
-        const int target = wstart + kWinTarget;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (hist[mid] >= target) hi = mid; else lo = mid + 1;
-        }
-        const int bnext = lo;
-        const int wend = bnext < B ? hist[bnext] : (int)n;
-        const int wcnt = wend - wstart;
-        if (tid == 0) counts[1] = 0;
-        __syncthreads();   // every thread has read hist before the pass moves the cursors
-        {
-            // each wavefront walks its own slices of kCompPerWave samples
-            const int lane = lane_id();
-            const int wave = uniform((int32_t)wave_id());
-            const int nwaves = nt / kWave;
-            int* wc = comp + wave * kCompPerWave;
-            for (int64_t base = (int64_t)wave * kCompPerWave; base < n;
-                 base += (int64_t)nwaves * kCompPerWave) {
-                int bv[kCompU];
-                int cnt = 0;
-#pragma unroll
-                for (int u = 0; u < kCompU; ++u) {
-                    const int64_t j = base + u * kWave + lane;
-                    bv[u] = j < n ? (int)bk[j] : -1;
-                    cnt += (bv[u] >= bfirst && bv[u] < bnext) ? 1 : 0;
-                }
-                const int incl = wave_inclusive_scan(cnt);
-                const int total = __shfl(incl, kWave - 1, kWave);
-                int o = incl - cnt;
-#pragma unroll
-                for (int u = 0; u < kCompU; ++u) {
-                    if (bv[u] >= bfirst && bv[u] < bnext) wc[o++] = (int)(base + u * kWave + lane);
-                }
-                wave_sync_lds();
-                for (int e = lane; e < total; e += kWave) {
-                    const int j = wc[e];
-                    const double ph = fold_phase(t[j], P);
-                    const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1) - wstart;
-                    sphi[pos] = ph;
-                    sidx[pos] = (int32_t)j;
-                }
-                wave_sync_lds();
+    // the windows: buckets wb[w] .. wb[w + 1] - 1, wb[w + 1] the first bucket starting at or
+    // past ws[w] + kWinTarget, so that a window holds fewer than kWinCap samples
+    if (tid == 0) {
+        int K = 0, bfirst = 0, wstart = 0;
+        while (bfirst < B && K < kMaxWin) {
+            int lo = bfirst + 1, hi = B;
+            const int target = wstart + kWinTarget;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (hist[mid] >= target) hi = mid; else lo = mid + 1;
             }
+            wb[K] = bfirst;
+            ws[K] = wstart;
+            ++K;
+            bfirst = lo;
+            wstart = lo < B ? hist[lo] : (int)n;
+        }
+        wb[K] = B;
+        ws[K] = (int)n;
+        counts[2] = K;
+        counts[3] = bfirst < B ? 1 : 0;   // more windows needed than the table holds
+    }
+    __syncthreads();
+    if (counts[3] != 0) return false;
+    const int K = counts[2];
+    for (int b = tid; b < B; b += nt) {
+        int lo = 0, hi = K;   // last w with wb[w] <= b
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (wb[mid] <= b) lo = mid; else hi = mid;
+        }
+        wtab[b] = (uint8_t)lo;
+    }
+    for (int w = tid; w < K; w += nt) wcur[w] = ws[w];
+    __syncthreads();
+    // one pass: every sample's index into its window's range of `binned`
+    {
+        constexpr int UW = 8;
+        for (int64_t base = 0; base < n; base += (int64_t)UW * nt) {
+            int bv[UW];
+#pragma unroll
+            for (int u = 0; u < UW; ++u) {
+                const int64_t j = base + (int64_t)u * nt + tid;
+                bv[u] = j < n ? (int)bk[j] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < UW; ++u) {
+                const int64_t j = base + (int64_t)u * nt + tid;
+                if (bv[u] >= 0) binned[atomicAdd(&wcur[wtab[bv[u]]], 1)] = (int32_t)j;
+            }
+        }
+    }
+    __syncthreads();
+    for (int w = 0; w < K; ++w) {
+        const int bfirst = wb[w], bnext = wb[w + 1];
+        const int wstart = ws[w], wcnt = ws[w + 1] - wstart;
+        if (tid == 0) counts[1] = 0;
+        __syncthreads();
+        // stage the window's samples at their bucket positions
+        for (int e = tid; e < wcnt; e += nt) {
+            const int j = binned[wstart + e];
+            const double ph = fold_phase(t[j], P);
+            const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1) - wstart;
+            sphi[pos] = ph;
+            sidx[pos] = (int32_t)j;
         }
         __syncthreads();
         // hist[b] now holds the end of bucket b for the window's buckets
@@ -535,8 +548,6 @@ __device__ __forceinline__ bool wg_fold_sort_windows(const double* __restrict__ 
         __syncthreads();
         sink(wstart, wcnt);
         __syncthreads();
-        bfirst = bnext;
-        wstart = wend;
     }
     return true;
 }

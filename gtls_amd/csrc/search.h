@@ -167,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
     __shared__ __attribute__((aligned(16))) double s_phi[kWinCap];
     __shared__ int32_t s_idx[kWinCap];
     __shared__ int64_t s_mids[kWinMid];
-    __shared__ int s_comp[(kBlock / kWave) * kCompPerWave];
+    __shared__ uint8_t s_wtab[kMaxBuckets];
+    __shared__ int s_wb[kMaxWin + 1], s_ws[kMaxWin + 1], s_wcur[kMaxWin];
     __shared__ int red_i[kBlock / kWave];
     __shared__ int s_counts[4];
     __shared__ double red_d[kBlock / kWave];
@@ -223,8 +224,8 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
             const R2* uw = p.uw0 + pt0;
             double carry = 0.0;
             const bool windowed = wg_fold_sort_windows(
-                p.t + pt0, n, P, p.negative_time != 0, bk, hist, red_i, s_counts, s_phi, s_idx,
-                s_mids, s_comp, [&](int wstart, int wcnt) {
+                p.t + pt0, n, P, p.negative_time != 0, bk, perm, hist, red_i, s_counts, s_phi,
+                s_idx, s_mids, s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                     gather_scan<Real>(uw, wstart, wcnt, [&](int64_t j) { return s_idx[j]; },
                                       constw, ub, wpl, D, carry, red_d);
                 });
@@ -449,7 +450,8 @@ __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int6
     __shared__ __attribute__((aligned(16))) double s_phi[kWinCap];
     __shared__ int32_t s_idx[kWinCap];
     __shared__ int64_t s_mids[kWinMid];
-    __shared__ int s_comp[(kBlock / kWave) * kCompPerWave];
+    __shared__ uint8_t s_wtab[kMaxBuckets];
+    __shared__ int s_wb[kMaxWin + 1], s_ws[kMaxWin + 1], s_wcur[kMaxWin];
     __shared__ int red_i[kBlock / kWave];
     __shared__ int s_counts[4];
     char* slot = slots + (int64_t)blockIdx.x * fold_sort_slot_bytes(n);
@@ -461,8 +463,8 @@ __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int6
     for (int64_t pi = blockIdx.x; pi < n_periods; pi += gridDim.x) {
         int32_t* out = perm_out + pi * n;
         const bool windowed = wg_fold_sort_windows(
-            t, n, periods[pi], neg != 0, bk, hist, red_i, s_counts, s_phi, s_idx, s_mids, s_comp,
-            [&](int wstart, int wcnt) {
+            t, n, periods[pi], neg != 0, bk, tmp_id, hist, red_i, s_counts, s_phi, s_idx, s_mids,
+            s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                 for (int j = threadIdx.x; j < wcnt; j += blockDim.x) out[wstart + j] = s_idx[j];
             });
         if (!windowed)
