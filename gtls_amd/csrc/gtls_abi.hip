@@ -118,9 +118,9 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     L.n_win = (mode == GTLS_MODE_FP64) ? ctx->win_fp64 : ctx->win_fp32;
     L.slots = off;
     off = align_up(off + L.slot_bytes * L.n_slots);
-    // per period of a chunk: gathered samples N + widest width of the wrap, plus up to
-    // another max_width read (and discarded) past them by the window loops
-    L.xstride = max_points + 2 * max_width + 128;
+    // per period of a chunk: gathered samples N + widest width of the wrap, plus the zeros
+    // the window loops read past them (up to another max_width + 96, or 416)
+    L.xstride = max_points + 2 * max_width + 512;
     L.kmax = b->max_period_widths;
     const int64_t per_period =
         L.xstride * (2 * real + 8) + (int64_t)sizeof(PeriodMeta) + L.kmax * (int64_t)sizeof(ArgMin);

@@ -250,8 +250,11 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
                 D[j] = (double)laps * carry + D[jj];
             }
             // zeros past the sequence, up to where the window loops read (below
-            // N + d + q + 64 <= N + 2 dmax + 64): those samples meet zero template entries
-            const int64_t zend = min(p.xstride, n + 2 * (int64_t)dmax + 96);
+            // N + d + q + 64 <= N + 2 dmax + 64 for the diagonal-sum tiles, below
+            // N + d + 15 q + 32 <= N + dmax + 377 for the Toeplitz tiles, q < 24): those
+            // samples meet zero template entries
+            const int64_t zend =
+                min(p.xstride, n + max(2 * (int64_t)dmax + 96, (int64_t)dmax + 416));
             for (int64_t j = np_ + tid; j < zend; j += kBlock) {
                 ub[j] = (Real)0;
                 if (!constw) wpl[j] = (Real)0;
@@ -343,7 +346,19 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
     const int32_t F = d / q;
     ArgMin w;
     if constexpr (sizeof(Real) == 4) {
-        if (p.use_mfma && q >= 4 && F + 1 <= 16) {
+        if (p.use_mfma && q < kToepQ) {
+            const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
+            if (constw)
+                w = toep_width<true>((const float*)ub, nullptr, (const float*)(p.tmpl1 + toff),
+                                     nullptr, D, a);
+            else
+                w = toep_width<false>((const float*)vw.v, (const float*)vw.w,
+                                      (const float*)(p.tmpl1 + toff),
+                                      (const float*)(p.tmplsq + toff), D, a);
+            if (argmin_less(w, best)) best = w;
+            return best;
+        }
+        if (p.use_mfma && F + 1 <= 16) {
             const int64_t co = uniform(chunk_offset(p.tmpl_off[wg], wg));
             if (constw)
                 w = window_sums_mfma<true>((const float*)ub, nullptr, p.tmplc1 + co, nullptr,

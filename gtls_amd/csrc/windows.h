@@ -39,7 +39,7 @@ __device__ __forceinline__ void consider(ArgMin& b, double v, int32_t k, int32_t
     ArgMin c{v, k, s};
     if (argmin_less(c, b)) b = c;
 }
-constexpr int kTmplPad = 16;   // zero samples before and after each template row
+constexpr int kTmplPad = 384;  // zero samples before and after each template row
 
 // Template rows are read through the constant address space: they are never written by
 // the kernel, and uniform loads from it compile to scalar (SMEM) loads.
@@ -383,6 +383,122 @@ __device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
             }
         }
         ring0 = ring0 + 64 >= kRingRows ? ring0 + 64 - kRingRows : ring0 + 64;
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------------------
+// Short strides (q < kToepQ) on the matrix cores as Toeplitz tiles: with rows i of a tile
+// starting at sample q (256 T + 16 i) and column j holding the template shifted by q j,
+//     C(i, j) = sum_k X[q (256 T + 16 i) + k] S[k - q j] = A(candidate 256 T + 16 i + j),
+// k running over the d + 15 q taps where a column can be non-zero. No diagonal sums: the
+// result register v of lane l is candidate 256 T + 16 (4 (l / 16) + v) + l % 16. Four
+// tiles (1024 candidates) share each template operand. The template rows carry kTmplPad
+// (>= 15 q + 32) zeros on both sides, so taps past a shifted template read zeros, and the
+// sample plane is zero past the data (reads stay below N + d + 15 q + 32).
+// ---------------------------------------------------------------------------------------
+constexpr int kToepQ = 24;
+
+template <bool ConstW>
+__device__ __forceinline__ ArgMin toep_width(const float* __restrict__ X,
+                                             const float* __restrict__ X2,
+                                             const float* __restrict__ T,
+                                             const float* __restrict__ T2,
+                                             const double* __restrict__ D, WidthArgs a) {
+    constexpr int M = 4, GS = 4 * M;
+    const int lane = lane_id();
+    const int kk = lane >> 4, col = lane & 15;
+    const int32_t d = a.d, q = a.q;
+    const int64_t K = a.K;
+    const int32_t L = d + 15 * q;                       // taps
+    const int32_t ngroups = (((L + GS - 1) / GS) + 1) & ~1;
+    const int64_t nsuper = (K + 1023) / 1024;
+    ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
+    // B operand: template value at tap k0 + 4 kk + s shifted by q col
+    const float* tb = T + M * kk - q * col;
+    const float* tb2 = T2 + M * kk - q * col;
+    for (int64_t st = 0; st < nsuper; ++st) {
+        const float* xa[4];
+        const float* xa2[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            // row `col` of tile t: candidates 1024 st + 256 t + 16 col .. + 15; rows wholly
+            // past the last candidate read sample 0 (results unused)
+            const int64_t m0 = 1024 * st + 256 * t + 16 * col;
+            const int64_t off = (m0 < K ? m0 * q : 0) + M * kk;
+            xa[t] = X + off;
+            if constexpr (!ConstW) xa2[t] = X2 + off;
+        }
+        mfma_v4 acc[4], acc2[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+            acc2[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+        }
+        FV<M> a0[4], b0, a1[4], b1;
+        auto load = [&](int32_t g, const float* const(&xs)[4], const float* tbase,
+                        FV<M>(&xa_)[4], FV<M>& xb) {
+            const int32_t o = GS * min(g, ngroups - 1);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) xa_[t] = ldm<M>(xs[t] + o);
+            xb = ldm<M>(tbase + o);
+        };
+        auto mma = [&](mfma_v4(&ac)[4], const FV<M>(&xa_)[4], const FV<M>& xb) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    ac[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa_[t].v[i], xb.v[i], ac[t],
+                                                                 0, 0, 0);
+            }
+        };
+        // A sums (and B sums with the weights), loads one group ahead of the MFMAs
+        for (int pass = 0; pass < (ConstW ? 1 : 2); ++pass) {
+            const float* const(&xs)[4] = pass == 0 ? xa : xa2;
+            const float* tbase = pass == 0 ? tb : tb2;
+            mfma_v4(&ac)[4] = pass == 0 ? acc : acc2;
+            load(0, xs, tbase, a0, b0);
+            for (int32_t g = 0; g < ngroups; g += 2) {
+                load(g + 1, xs, tbase, a1, b1);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(ac, a0, b0);
+                __builtin_amdgcn_sched_barrier(0);
+                load(g + 2, xs, tbase, a0, b0);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(ac, a1, b1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // the statistic of the lane's 16 candidates
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            double dlo[4], dhi[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t m = 1024 * st + 256 * t + 16 * (4 * kk + v) + col;
+                const int64_t s = (m < K ? m : 0) * q;
+                dhi[v] = D[s + d - 1];
+                dlo[v] = s ? D[s - 1] : 0.0;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t m = 1024 * st + 256 * t + 16 * (4 * kk + v) + col;
+                if (m < K) {
+                    const int32_t s = (int32_t)(m * q);
+                    const double mm = (dhi[v] - dlo[v]) * a.inv_d;
+                    if (mm > a.depth_min) {
+                        const float rs = (float)(mm * a.os2);
+                        const float sa = acc[t][v];
+                        const double delta =
+                            ConstW ? a.wc * ((double)(rs * rs) * a.ssk - 2.0 * (double)rs * (double)sa)
+                                   : (double)(rs * rs * acc2[t][v] - 2.f * rs * sa);
+                        consider(best, a.chi2f + delta, a.k, s);
+                    } else {
+                        consider(best, a.sentinel, a.k, s);
+                    }
+                }
+            }
+        }
     }
     return best;
 }
