@@ -62,6 +62,22 @@ def algorithmic_bytes(plan, real_bytes=4):
     return float(np.sum(((8 + 2 * real_bytes) * plan.n + 2 * real_bytes * wmax + 16) * ok))
 
 
+def profiled_traffic(args):
+    """HBM bytes of one search call as profiled (FETCH_SIZE and WRITE_SIZE, copy-scaled) for
+    this workload, read from the committed profile summary; None for other workloads."""
+    if (args.curves, args.batch, args.seed) != (4096, 512, 4):
+        return None, None
+    path = os.path.join(ROOT, "profiles", "search_fp32c_c4_batch0_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def stratified(work, k):
     """k curve indices spread evenly over the sorted work distribution."""
     order = np.argsort(work, kind="stable")
@@ -175,10 +191,13 @@ def main():
     bytes_ = np.array([algo[c].sum() for c in step_curves])
     achieved = float(np.mean(flops / (search_ms * 1e-3)) / 1e12)
     hbm_achieved = float(np.mean(bytes_ / (search_ms * 1e-3)) / 1e9)
+    traffic, traffic_src = profiled_traffic(args)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3),
                 "peak": FP32_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_MATRIX_PEAK_TFLOPS, 5), "traffic": None,
-                "kernel": "tls_search_kernel<float>",
+                "frac": round(achieved / FP32_MATRIX_PEAK_TFLOPS, 5), "traffic": traffic,
+                "traffic_source": (traffic_src + " (batch 0)") if traffic_src else None,
+                "kernel": "gtls_search_batched: prep_kernel + windows_kernel + reduce_kernel "
+                          "over every period chunk",
                 "launch_ms": round(float(search_ms.mean()), 3),
                 "launch_ms_per_step": [round(float(x), 3) for x in search_ms],
                 "algorithmic_flops_per_launch": float(flops.mean()),

@@ -1,5 +1,5 @@
 """Batched search: many light curves, each with its own period grid and template table, in
-one persistent launch (gtls_search_batched).
+one gtls_search_batched call per size group.
 
 Curves are packed in CSR layout (points, periods, widths, template samples), largest
 first so that the device work queue hands out the longest items early, and the

@@ -20,6 +20,7 @@ struct gtls_ctx {
     int win_fp64 = 0, win_fp32 = 0;     // resident workgroups of windows_kernel
     int slots_sort = 0;
     int use_mfma = 1;
+    int toep_q = gtls::kToepQ;
     int64_t chunk_bytes = 4LL << 30;    // budget of one chunk's gathered samples
     std::string err;
 };
@@ -235,6 +236,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.depth_min = b->depth_min;
     p.negative_time = b->negative_time;
     p.use_mfma = ctx->use_mfma;
+    p.toep_q = ctx->toep_q;
     p.max_points = b->max_points;
     p.max_width = b->max_width > 0 ? b->max_width : 1;
     p.kmax = (int32_t)L.kmax;
@@ -300,6 +302,8 @@ int gtls_create(int device, void* stream, gtls_ctx** out) {
     ctx->stream = (hipStream_t)stream;
     ctx->n_cu = prop.multiProcessorCount;
     if (const char* e = std::getenv("GTLS_MFMA")) ctx->use_mfma = std::atoi(e) != 0;
+    if (const char* e = std::getenv("GTLS_TOEP_Q"))
+        ctx->toep_q = std::max(0, std::min(kToepQ, std::atoi(e)));
     if (const char* e = std::getenv("GTLS_CHUNK_MB")) {
         const long long mb = std::atoll(e);
         if (mb > 0) ctx->chunk_bytes = mb << 20;

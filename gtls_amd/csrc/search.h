@@ -63,6 +63,7 @@ struct SearchParams {
     double depth_min;
     int32_t negative_time;
     int32_t use_mfma;     // float32: window sums on the matrix cores
+    int32_t toep_q;       // float32: strides below this take the Toeplitz tiles (<= kToepQ)
     int32_t max_points;   // every curve must have n <= max_points
     int32_t max_width;    // every searched width must be <= max_width
     int32_t kmax;         // items per period in the window stage (>= widths per period)
@@ -346,7 +347,7 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
     const int32_t F = d / q;
     ArgMin w;
     if constexpr (sizeof(Real) == 4) {
-        if (p.use_mfma && q < kToepQ) {
+        if (p.use_mfma && q < p.toep_q) {
             const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
             if (constw)
                 w = toep_width<true>((const float*)ub, nullptr, (const float*)(p.tmpl1 + toff),

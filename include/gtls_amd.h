@@ -20,7 +20,7 @@
  *       (GPUFun.cu:380-436) and cp.argmin (core.py:866-875); the no-skip variants of the
  *       re-search (core.py:1001-1247, GPUFun.cu:817-871) are the same call with
  *       skip_point = GTLS_SKIP_NONE. The batched form runs many light curves, each with
- *       its own period grid and width table, in one persistent launch.
+ *       its own period grid and width table, in one call.
  *   gtls_fold_sort
  *       foldFast + argsort (GPUFun.cu:28-39, core.py:775-777): the stable phase-order
  *       permutation of every trial period, exposed for bit-exact checks.

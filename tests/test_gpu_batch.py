@@ -1,4 +1,4 @@
-"""Batched search (one persistent launch over many curves) and device spectra, through
+"""Batched search (one call over many curves) and device spectra, through
 the C ABI, against the FP64 oracle per curve and the host spectra."""
 import ctypes as ct
 
