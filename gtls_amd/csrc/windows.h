@@ -200,7 +200,9 @@ namespace gtls {
 // The K (b) loop runs in groups of 4 M samples = M MFMA steps; within a group, step s uses
 // sample b0 + M (l / 16) + s in K slot l / 16, so each lane's operands for a group are M
 // consecutive floats. The loads of the next group are issued before the MFMAs of the
-// current one.
+// current one, and the first group of the next sweep (the B plane, or the next
+// super-tile) before the last MFMAs of this one, so its latency passes during the
+// diagonal sums.
 // ---------------------------------------------------------------------------------------
 typedef float mfma_v4 __attribute__((ext_vector_type(4)));
 constexpr int kRingRows = 80;     // 64 rows of a super-tile + up to 15 look-ahead rows
@@ -233,76 +235,133 @@ __device__ __forceinline__ int ring_wrap(int r) {
 __host__ __device__ inline int32_t mfma_m(int32_t q) { return q >= 48 ? 8 : (q >= 16 ? 4 : 2); }
 
 // Chunked template rows of the matrix-core path: for width w with stride q, F = d / q and
-// q_pad = q rounded up to an even number of K groups (8 mfma_m(q) samples), chunk c <= F
-// sits at chunk_offset(w) + c q_pad, holds S[c q + b] for b < its length (q, or d - F q
-// for c = F) and zeros up to q_pad. Columns c > F read a block of zeros instead.
-// (F + 1) q_pad <= 2 d + 1008 for F <= 15, so the offsets below leave room for every row.
+// q_pad = q rounded up to whole K groups (4 mfma_m(q) samples), chunk c <= F sits at
+// chunk_offset(w) + c q_pad, holds S[c q + b] for b < its length (q, or d - F q for
+// c = F) and zeros up to q_pad. Columns c > F read a block of zeros instead.
+// (F + 1) q_pad <= 2 d + 1024 for F <= 15, so the offsets below leave room for every row.
 __host__ __device__ inline int64_t chunk_offset(int64_t tmpl_off, int64_t w) {
     return 2 * tmpl_off + 1024 * w;
 }
 __host__ __device__ inline int32_t chunk_pad(int32_t q) {
-    const int32_t g = 8 * mfma_m(q);
+    const int32_t g = 4 * mfma_m(q);
     return (q + g - 1) / g * g;
 }
 
-// The 64 G rows of super-tile st for one (sample plane X, chunked template C) pair,
-// written to the ring. A operand X[row q + b]: samples b >= q meet zero template entries
-// and rows past the data read block 0, so the sample plane needs only finite values (the
-// gather zero-fills its tail); B operand C[c q_pad + b]. No masking in the loop: the loads
-// of the next group stay in flight while the MFMAs of the current one issue.
+// Operand addresses of one sweep (the 64 G rows of super-tile st for one sample plane X
+// and chunked template C): the lane's A pointers (4 tiles) and B pointer, and the group
+// count. A operand X[row q + b]: samples b >= q meet zero template entries and rows past
+// the data read block 0, so the sample plane needs only finite values (the gather
+// zero-fills its tail); B operand C[c q_pad + b], columns c > F read the zero block Z.
 template <int M>
-__device__ __forceinline__ void super_tile_rows(const float* __restrict__ X,
-                                                const float* __restrict__ C,
-                                                const float* __restrict__ Z, int32_t q,
-                                                int32_t F, int64_t rows, int64_t st, int ring0,
-                                                float* ring) {
-    constexpr int GS = 4 * M;
+struct GSweep {
+    const float* X;      // sample plane (wave-uniform)
+    const float* C;      // template rows (wave-uniform)
+    uint32_t xo[4];      // lane's A operand byte offsets from X, one per tile
+    int32_t co;          // lane's B operand byte offset from C
+};
+
+template <int M>
+__device__ __forceinline__ GSweep<M> g_sweep(const float* __restrict__ X,
+                                             const float* __restrict__ C,
+                                             const float* __restrict__ Z, int32_t q, int32_t F,
+                                             int64_t rows, int64_t st) {
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
-    const float* tc = (col <= F) ? C + (int64_t)col * chunk_pad(q) : Z;
-    // an even number of groups: the last one of an odd count meets zero template entries
-    const int32_t ngroups = (((q + GS - 1) / GS) + 1) & ~1;
-    const float* xa_[4];
+    GSweep<M> w;
+    w.X = X;
+    w.C = C;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int64_t row = 64 * st + 16 * t + col;
-        xa_[t] = X + (row < rows ? row * q : 0) + M * kk;
+        w.xo[t] = (uint32_t)(4 * ((row < rows ? row * q : 0) + M * kk));
     }
-    tc += M * kk;
-    mfma_v4 acc[4];
+    w.co = (int32_t)(4 * ((col <= F) ? (int64_t)col * chunk_pad(q) + M * kk
+                                     : (int64_t)(Z - C) + M * kk));
+    return w;
+}
+
+// One K group's operands of a sweep (M MFMA steps for each of the 4 tiles).
+template <int M>
+struct GSet {
+    FV<M> a[4];
+    FV<M> b;
+};
+
+template <int M>
+__device__ __forceinline__ void g_load(const GSweep<M>& w, int32_t g, GSet<M>& o) {
+    const uint32_t off = 16 * M * g;
+    const char* x = (const char*)w.X;
+    const char* c = (const char*)w.C;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) o.a[t] = ldm<M>((const float*)(x + (w.xo[t] + off)));
+    o.b = ldm<M>((const float*)(c + (int64_t)(w.co + (int32_t)off)));
+}
+
+// K groups of a sweep over `len` taps (4 M samples per group; taps past `len` meet zero
+// template entries).
+__device__ __forceinline__ int32_t k_groups(int32_t len, int M) {
+    return (len + 4 * M - 1) / (4 * M);
+}
+
+// One sweep into acc (zeroed here): `cur` with its group 0 already in s0; on return s0
+// holds group 0 of the sweep `nxt`, whose loads stay in flight through whatever the caller
+// does next. Two explicit operand sets with straight-line pairs of groups: each MFMA
+// block waits only for its own set while the other set's loads are in flight (the
+// scheduling barriers keep each load block ahead of the MFMA block before it). Within a
+// group, step i takes sample 4 M g + M kk + i in K slot kk, so each lane's operands are M
+// consecutive floats.
+template <int M>
+__device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[4], const GSweep<M>& cur,
+                                          const GSweep<M>& nxt, int32_t ng, GSet<M>& s0,
+                                          GSet<M>& s1) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
-    // two explicit operand sets, loads always issued (the last group is fetched again at
-    // the end), so the loop body is straight-line and each MFMA block waits only for its
-    // own set while the other set's loads are in flight
-    FV<M> a0[4], b0, a1[4], b1;
-    auto load = [&](int32_t g, FV<M>(&xa)[4], FV<M>& xb) {
-        const int32_t o = GS * min(g, ngroups - 1);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) xa[t] = ldm<M>(xa_[t] + o);
-        xb = ldm<M>(tc + o);
-    };
-    auto mma = [&](const FV<M>(&xa)[4], const FV<M>& xb) {
+    auto mma = [&](const GSet<M>& o) {
 #pragma unroll
         for (int i = 0; i < M; ++i) {
 #pragma unroll
             for (int t = 0; t < 4; ++t)
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[t].v[i], xb.v[i], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[t].v[i], o.b.v[i], acc[t],
+                                                             0, 0, 0);
         }
     };
-    // the scheduling barriers keep each load block ahead of the MFMA block before it, so
-    // the loads of one set overlap the MFMAs of the other
-    load(0, a0, b0);
-    for (int32_t g = 0; g < ngroups; g += 2) {
-        load(g + 1, a1, b1);
+    int32_t g = 0;
+    for (; g + 1 < ng; g += 2) {
+        g_load(cur, g + 1, s1);
         __builtin_amdgcn_sched_barrier(0);
-        mma(a0, b0);
+        mma(s0);
         __builtin_amdgcn_sched_barrier(0);
-        load(g + 2, a0, b0);
+        // group g + 2 of this sweep, or group 0 of the next one (addresses selected, so
+        // the loads are unconditional)
+        const bool more = g + 2 < ng;
+        GSweep<M> w;
+        w.X = more ? cur.X : nxt.X;
+        w.C = more ? cur.C : nxt.C;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w.xo[t] = more ? cur.xo[t] : nxt.xo[t];
+        w.co = more ? cur.co : nxt.co;
+        g_load(w, more ? g + 2 : 0, s0);
         __builtin_amdgcn_sched_barrier(0);
-        mma(a1, b1);
+        mma(s1);
         __builtin_amdgcn_sched_barrier(0);
     }
+    if (g < ng) {   // an odd group count: the last group is in s0
+        mma(s0);
+        __builtin_amdgcn_sched_barrier(0);
+        g_load(nxt, 0, s0);
+    }
+}
+
+// The sweep `cur` of a super-tile, written to the ring (see run_sweep).
+template <int M>
+__device__ __forceinline__ void super_tile_rows(const GSweep<M>& cur, const GSweep<M>& nxt,
+                                                int32_t ng, GSet<M>& s0, int ring0,
+                                                float* ring) {
+    const int lane = lane_id();
+    const int kk = lane >> 4, col = lane & 15;
+    mfma_v4 acc[4];
+    GSet<M> s1;
+    run_sweep<M>(acc, cur, nxt, ng, s0, s1);
     // rows 16 t + 4 kk .. + 3 of column col: one 16-byte store (ring0 is a multiple of 16
     // and kRingRows of 4, so the four rows never wrap apart)
 #pragma unroll
@@ -339,7 +398,12 @@ __device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
     const int64_t rows = K + F;
     const int64_t nsuper = (rows + 63) / 64;
     ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
+    const int32_t ng = k_groups(q, M);
     int ring0 = 0;   // ring row of G row 64 T
+    // sweeps in order (A plane, then B plane, of each super-tile); the first group of each
+    // sweep is loaded during the one before (past the last super-tile: row block 0 again)
+    GSet<M> ops;
+    g_load(g_sweep<M>(X, C, Z, q, F, rows, 0), 0, ops);
     for (int64_t st = 0; st < nsuper; ++st) {
         // the window-mean prefix sums of this super-tile's candidates, requested early
         const int64_t kc = 64 * st - F + lane;
@@ -350,8 +414,15 @@ __device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
             dhi = D[s + d - 1];
             dlo = s ? D[s - 1] : 0.0;
         }
-        super_tile_rows<M>(X, C, Z, q, F, rows, st, ring0, ringA);
-        if constexpr (!ConstW) super_tile_rows<M>(X2, C2, Z, q, F, rows, st, ring0, ringB);
+        const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
+        const GSweep<M> wa = g_sweep<M>(X, C, Z, q, F, rows, st);
+        if constexpr (ConstW) {
+            super_tile_rows<M>(wa, g_sweep<M>(X, C, Z, q, F, rows, stn), ng, ops, ring0, ringA);
+        } else {
+            const GSweep<M> wb = g_sweep<M>(X2, C2, Z, q, F, rows, st);
+            super_tile_rows<M>(wa, wb, ng, ops, ring0, ringA);
+            super_tile_rows<M>(wb, g_sweep<M>(X, C, Z, q, F, rows, stn), ng, ops, ring0, ringB);
+        }
         wave_sync_lds();
         // diagonal sums: all 16 reads issue before the first add, columns past F masked
         float va[16], vb[16];
@@ -405,69 +476,41 @@ __device__ __forceinline__ ArgMin toep_width(const float* __restrict__ X,
                                              const float* __restrict__ T,
                                              const float* __restrict__ T2,
                                              const double* __restrict__ D, WidthArgs a) {
-    constexpr int M = 4, GS = 4 * M;
+    constexpr int M = 4;
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
     const int32_t d = a.d, q = a.q;
     const int64_t K = a.K;
-    const int32_t L = d + 15 * q;                       // taps
-    const int32_t ngroups = (((L + GS - 1) / GS) + 1) & ~1;
+    const int32_t ng = k_groups(d + 15 * q, M);         // d + 15 q taps
     const int64_t nsuper = (K + 1023) / 1024;
     ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
-    // B operand: template value at tap k0 + 4 kk + s shifted by q col
-    const float* tb = T + M * kk - q * col;
-    const float* tb2 = T2 + M * kk - q * col;
-    for (int64_t st = 0; st < nsuper; ++st) {
-        const float* xa[4];
-        const float* xa2[4];
+    // sweep of step st over one plane: row `col` of tile t covers candidates
+    // 1024 st + 256 t + 16 col .. + 15 (rows wholly past the last candidate read sample 0,
+    // results unused); B operand: template value at tap 4 M kk + s shifted by q col
+    auto sweep = [&](const float* P, const float* Tp, int64_t st) {
+        GSweep<M> w;
+        w.X = P;
+        w.C = Tp;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            // row `col` of tile t: candidates 1024 st + 256 t + 16 col .. + 15; rows wholly
-            // past the last candidate read sample 0 (results unused)
             const int64_t m0 = 1024 * st + 256 * t + 16 * col;
-            const int64_t off = (m0 < K ? m0 * q : 0) + M * kk;
-            xa[t] = X + off;
-            if constexpr (!ConstW) xa2[t] = X2 + off;
+            w.xo[t] = (uint32_t)(4 * ((m0 < K ? m0 * q : 0) + M * kk));
         }
-        mfma_v4 acc[4], acc2[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
-            acc2[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
-        }
-        FV<M> a0[4], b0, a1[4], b1;
-        auto load = [&](int32_t g, const float* const(&xs)[4], const float* tbase,
-                        FV<M>(&xa_)[4], FV<M>& xb) {
-            const int32_t o = GS * min(g, ngroups - 1);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) xa_[t] = ldm<M>(xs[t] + o);
-            xb = ldm<M>(tbase + o);
-        };
-        auto mma = [&](mfma_v4(&ac)[4], const FV<M>(&xa_)[4], const FV<M>& xb) {
-#pragma unroll
-            for (int i = 0; i < M; ++i) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    ac[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa_[t].v[i], xb.v[i], ac[t],
-                                                                 0, 0, 0);
-            }
-        };
-        // A sums (and B sums with the weights), loads one group ahead of the MFMAs
-        for (int pass = 0; pass < (ConstW ? 1 : 2); ++pass) {
-            const float* const(&xs)[4] = pass == 0 ? xa : xa2;
-            const float* tbase = pass == 0 ? tb : tb2;
-            mfma_v4(&ac)[4] = pass == 0 ? acc : acc2;
-            load(0, xs, tbase, a0, b0);
-            for (int32_t g = 0; g < ngroups; g += 2) {
-                load(g + 1, xs, tbase, a1, b1);
-                __builtin_amdgcn_sched_barrier(0);
-                mma(ac, a0, b0);
-                __builtin_amdgcn_sched_barrier(0);
-                load(g + 2, xs, tbase, a0, b0);
-                __builtin_amdgcn_sched_barrier(0);
-                mma(ac, a1, b1);
-                __builtin_amdgcn_sched_barrier(0);
-            }
+        w.co = 4 * (M * kk - q * col);
+        return w;
+    };
+    mfma_v4 acc[4], acc2[4];
+    GSet<M> s0, s1;
+    g_load(sweep(X, T, 0), 0, s0);
+    for (int64_t st = 0; st < nsuper; ++st) {
+        const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
+        // A sums (and B sums with the weights); the first group of the next sweep is
+        // loaded during this one
+        if constexpr (ConstW) {
+            run_sweep<M>(acc, sweep(X, T, st), sweep(X, T, stn), ng, s0, s1);
+        } else {
+            run_sweep<M>(acc, sweep(X, T, st), sweep(X2, T2, st), ng, s0, s1);
+            run_sweep<M>(acc2, sweep(X2, T2, st), sweep(X, T, stn), ng, s0, s1);
         }
         // the statistic of the lane's 16 candidates
 #pragma unroll
