@@ -388,6 +388,7 @@ __device__ __forceinline__ void wg_fold_sort(const double* __restrict__ t, int64
 constexpr int kWinCap = 2816;                        // samples one window stages in LDS
 constexpr int kWinTarget = kWinCap - kWaveSortMax;   // window size before its last bucket
 constexpr int kWinMid = kWinCap / 17 + 1;            // buckets of 17+ keys in one window
+constexpr int kWinPer = (kWinCap + 255) / 256;       // staged samples per thread (256 threads)
 constexpr int kMaxWin = 255;                         // windows of one period (uint8 table)
 
 // Windowed stable phase sort of the n samples of t for period P (see the file comment).
@@ -396,6 +397,7 @@ constexpr int kMaxWin = 255;                         // windows of one period (u
 // sidx (kWinCap doubles / ints), mids (kWinMid int64), wtab (kMaxBuckets bytes: window of
 // each bucket), wb / ws (kMaxWin + 1 ints: first bucket / first position of each window),
 // wcur (kMaxWin ints).
+// Runs with 256 threads (kWinPer staged samples per thread).
 // sink(wstart, wcnt) is called by every thread for each window: the samples of sorted
 // positions wstart .. wstart + wcnt - 1 sit in sphi / sidx[0 .. wcnt); it may use
 // barriers. Returns false (every thread) when a bucket holds more than kWaveSortMax keys
@@ -509,13 +511,27 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         const int wstart = ws[w], wcnt = ws[w + 1] - wstart;
         if (tid == 0) counts[1] = 0;
         __syncthreads();
-        // stage the window's samples at their bucket positions
-        for (int e = tid; e < wcnt; e += nt) {
-            const int j = binned[wstart + e];
-            const double ph = fold_phase(t[j], P);
-            const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1) - wstart;
-            sphi[pos] = ph;
-            sidx[pos] = (int32_t)j;
+        // stage the window's samples at their bucket positions (indices, then times, of
+        // all of a thread's samples in flight before the first use)
+        {
+            int jj[kWinPer];
+            double tv[kWinPer];
+#pragma unroll
+            for (int u = 0; u < kWinPer; ++u) {
+                const int e = tid + u * nt;
+                jj[u] = e < wcnt ? binned[wstart + e] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < kWinPer; ++u) tv[u] = t[jj[u] >= 0 ? jj[u] : 0];
+#pragma unroll
+            for (int u = 0; u < kWinPer; ++u) {
+                if (jj[u] >= 0) {
+                    const double ph = fold_phase(tv[u], P);
+                    const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1) - wstart;
+                    sphi[pos] = ph;
+                    sidx[pos] = (int32_t)jj[u];
+                }
+            }
         }
         __syncthreads();
         // hist[b] now holds the end of bucket b for the window's buckets

@@ -122,14 +122,20 @@ __device__ __forceinline__ void gather_scan(const typename Pair<Real>::type* __r
         const int64_t wbase = base + (int64_t)wave * kWave * kScanPerThread;
         double u[kScanPerThread];
         double run = 0.0;
+        // all gathers of the chunk in flight before the first use
+        R2 g[kScanPerThread];
+#pragma unroll
+        for (int e = 0; e < kScanPerThread; ++e) {
+            const int64_t j = wbase + (int64_t)e * kWave + lane;
+            g[e] = uw[src(j < cnt ? j : 0)];
+        }
 #pragma unroll
         for (int e = 0; e < kScanPerThread; ++e) {
             const int64_t j = wbase + (int64_t)e * kWave + lane;
             Real uu = 0, ww = 0;
             if (j < cnt) {
-                const R2 a = uw[src(j)];
-                uu = a.x;
-                ww = a.y;
+                uu = g[e].x;
+                ww = g[e].y;
                 if (constw) {
                     ub[o0 + j] = uu;
                 } else {
