@@ -78,6 +78,54 @@ def profiled_traffic(args):
     return None, None
 
 
+def refine_leg(args, plans, batch0, skip):
+    """Gradient refinement of grid peaks: the 8 highest power peaks of each of the first
+    `refine_curves` curves of batch 0 seed a smooth limb-darkened transit model each, fitted
+    by Adam (`refine_steps` steps, float32 model on the HIP kernels)."""
+    import torch
+    from gtls_amd import refine as RF
+    bp, db = batch0
+    db.search("fp32c", skip)
+    spec = db.spectra(db.chi2, 3)
+    bks = bp.unpack(db.best_k.cpu().numpy())
+    bss = bp.unpack(db.best_s.cpu().numpy())
+    log("refine: seeding from the peaks of %d curves" % min(args.refine_curves, args.batch))
+    ts = time.perf_counter()
+    curves, thetas, mc, spans = [], [], [], []
+    for j in range(min(args.refine_curves, args.batch)):
+        p = plans[j]
+        th, _ = RF.seed_peaks(p.t, p.y, p.dy, p.periods, spec["power"][j], p.table.widths,
+                              bks[j], bss[j], 8)
+        curves.append((p.t, p.y, p.dy))
+        thetas.append(th)
+        mc += [len(curves) - 1] * len(th)
+        spans += [float(np.ptp(p.t))] * len(th)
+    thetas = np.concatenate(thetas)
+    mb = RF.ModelBatch.build(curves, mc, 0)
+    t_seed = time.perf_counter() - ts
+    log("refine: %d models over %d points seeded in %.1f s" % (mb.n_models, mb.slots, t_seed))
+    RF.Refiner(mb, thetas, spans).run(2)          # warm-up (kernels, allocator)
+    ref = RF.Refiner(mb, thetas, spans)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    first, last = ref.run(args.refine_steps,
+                          progress=lambda i: log("refine: step %d done" % i))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - ts
+    ratio = (last / first).cpu().numpy()
+    slots = mb.slots
+    out = {"curves": len(curves), "models": int(mb.n_models), "steps": args.refine_steps,
+           "model_points": slots, "seconds": round(dt, 3), "seed_seconds": round(t_seed, 2),
+           "ms_per_step": round(dt * 1e3 / args.refine_steps, 3),
+           "model_points_per_s": round(slots * args.refine_steps / dt, 1),
+           "curves_per_hour": round(len(curves) / dt * 3600.0, 1),
+           "chi2_ratio_median": float(np.median(ratio)),
+           "chi2_improved_models": "%d/%d" % (int(np.sum(ratio < 1)), len(ratio))}
+    log("refine: %d models, %d steps in %.2f s (%.2f ms/step), median chi2 ratio %.4f" % (
+        mb.n_models, args.refine_steps, dt, out["ms_per_step"], out["chi2_ratio_median"]))
+    return out
+
+
 def stratified(work, k):
     """k curve indices spread evenly over the sorted work distribution."""
     order = np.argsort(work, kind="stable")
@@ -107,6 +155,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--refine-curves", type=int, default=256)
+    ap.add_argument("--refine-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -246,6 +297,9 @@ def main():
                       "parity_vs_plain_max_rel_sde_err": float(max_dsde)})
         log("plain fp64: %.1f curves/h (sample %d), parity %d/%d %d/%d, max dSDE %.2e" % (
             plain_value, len(sample), same_p, len(sample), same_w, len(sample), max_dsde))
+
+    if not args.no_refine:
+        extra["refine"] = refine_leg(args, plans, batches[0], skip)
 
     cpu = None
     if not args.no_cpu:

@@ -11,6 +11,7 @@
 #include "gtls_amd.h"
 #include "search.h"
 #include "spectra.h"
+#include "transit_model.h"
 
 struct gtls_ctx {
     int device = 0;
@@ -21,7 +22,7 @@ struct gtls_ctx {
     int slots_sort = 0;
     int use_mfma = 1;
     int toep_q = gtls::kToepQ;
-    int64_t chunk_bytes = 4LL << 30;    // budget of one chunk's gathered samples
+    int64_t chunk_bytes = 0;            // budget of one chunk's gathered samples
     std::string err;
 };
 
@@ -301,6 +302,9 @@ int gtls_create(int device, void* stream, gtls_ctx** out) {
     ctx->device = device;
     ctx->stream = (hipStream_t)stream;
     ctx->n_cu = prop.multiProcessorCount;
+    // Each period chunk ends with the tails of its kernels, so chunks are made large: up
+    // to 32 GiB of per-period buffers, at most an eighth of the device memory.
+    ctx->chunk_bytes = std::min<int64_t>(32LL << 30, (int64_t)(prop.totalGlobalMem / 8));
     if (const char* e = std::getenv("GTLS_MFMA")) ctx->use_mfma = std::atoi(e) != 0;
     if (const char* e = std::getenv("GTLS_TOEP_Q"))
         ctx->toep_q = std::max(0, std::min(kToepQ, std::atoi(e)));
@@ -500,6 +504,56 @@ int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off
                        (const double*)med);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "spectra launch");
+    return GTLS_OK;
+}
+
+int gtls_transit_fwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
+                     const int32_t* model_curve, const int64_t* pt_off, const double* t,
+                     const int64_t* out_off, int64_t max_points, void* flux) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C)
+        return fail(ctx, GTLS_E_MODE, "unknown mode");
+    if (n_models < 0 || n_models > 65535 || max_points < 0)
+        return fail(ctx, GTLS_E_SHAPE, "transit model count must be in [0, 65535]");
+    if (n_models == 0 || max_points == 0) return GTLS_OK;
+    if (!theta || !model_curve || !pt_off || !t || !out_off || !flux)
+        return fail(ctx, GTLS_E_SHAPE, "null transit model argument");
+    DeviceGuard guard(ctx->device);
+    const unsigned bx = (unsigned)std::min<int64_t>((max_points + 1023) / 1024, 4096);
+    const dim3 grid(bx, (unsigned)n_models);
+    if (mode == GTLS_MODE_FP64)
+        hipLaunchKernelGGL(transit_fwd_kernel<double>, grid, dim3(256), 0, ctx->stream, theta,
+                           model_curve, pt_off, t, out_off, (double*)flux);
+    else
+        hipLaunchKernelGGL(transit_fwd_kernel<float>, grid, dim3(256), 0, ctx->stream, theta,
+                           model_curve, pt_off, t, out_off, (float*)flux);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "transit forward launch");
+    return GTLS_OK;
+}
+
+int gtls_transit_bwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
+                     const int32_t* model_curve, const int64_t* pt_off, const double* t,
+                     const int64_t* out_off, const void* grad_flux, double* grad_theta) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C)
+        return fail(ctx, GTLS_E_MODE, "unknown mode");
+    if (n_models < 0 || n_models > (1LL << 31) - 1)
+        return fail(ctx, GTLS_E_SHAPE, "bad transit model count");
+    if (n_models == 0) return GTLS_OK;
+    if (!theta || !model_curve || !pt_off || !t || !out_off || !grad_flux || !grad_theta)
+        return fail(ctx, GTLS_E_SHAPE, "null transit model argument");
+    DeviceGuard guard(ctx->device);
+    if (mode == GTLS_MODE_FP64)
+        hipLaunchKernelGGL(transit_bwd_kernel<double>, dim3((unsigned)n_models), dim3(256), 0,
+                           ctx->stream, theta, model_curve, pt_off, t, out_off,
+                           (const double*)grad_flux, grad_theta);
+    else
+        hipLaunchKernelGGL(transit_bwd_kernel<float>, dim3((unsigned)n_models), dim3(256), 0,
+                           ctx->stream, theta, model_curve, pt_off, t, out_off,
+                           (const float*)grad_flux, grad_theta);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "transit backward launch");
     return GTLS_OK;
 }
 

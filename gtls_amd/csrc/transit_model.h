@@ -1,0 +1,147 @@
+// Smooth limb-darkened transit model for the gradient refinement of grid peaks, with its
+// hand-written derivative. It has no counterpart in the reference: the reference stops at
+// the grid search and a host-side statistics pass (src/gputls/stats.py:233-488).
+//
+// Parameters of one model (theta, 6 float64 values): period P, mid-transit time T0,
+// duration D (days), depth delta, and Kipping's limb-darkening parameters q1, q2 in (0, 1)
+// (u1 = 2 sqrt(q1) q2, u2 = sqrt(q1) (1 - 2 q2)). For a time t:
+//     e = t - T0,  n = rint(e / P),  tau = e - n P             (float64: the fold)
+//     x = 2 tau / D                                            (chord coordinate)
+//     s = 1 - x^2,  W = sigmoid(kappa s),  mu = sqrt(softplus(kappa s) / kappa)
+//     I = 1 - u1 (1 - mu) - u2 (1 - mu)^2                      (quadratic law)
+//     f = 1 - delta W I
+// i.e. a central transit of a small planet across a quadratically limb-darkened star,
+// whose disk edge (|x| = 1) is smoothed over ~1/kappa so that f is differentiable in every
+// parameter. At the centre f = 1 - delta (to 1e-27); far from transit f = 1. Points with
+// kappa s below -kLogCut have W < 1e-26: their deficit and derivatives are set to zero.
+//
+// Derivatives (n held fixed: it is piecewise constant in P and T0):
+//     df/ddelta = -W I,   df/du1 = delta W (1 - mu),   df/du2 = delta W (1 - mu)^2,
+//     df/dx = -delta (W' I + W I'),  W' = kappa W (1 - W) (-2 x),
+//     I' = (u1 + 2 u2 (1 - mu)) dmu/dx,  dmu/dx = -W x / mu,
+//     dx/dT0 = -2 / D,  dx/dP = -2 n / D,  dx/dD = -x / D.
+#pragma once
+#include <cmath>
+#include "wg_primitives.h"
+
+namespace gtls {
+
+constexpr double kTransitKappa = 64.0;  // edge sharpness (1 / smoothing width in s)
+constexpr double kLogCut = 60.0;        // kappa s below -kLogCut: outside the model
+
+template <class Real>
+struct TransitPoint {
+    Real f;                      // model flux
+    Real dP, dT0, dD, ddelta, dq1, dq2;  // its partial derivatives
+};
+
+// Model flux of one point, and (when Grad) its six partial derivatives. theta as above;
+// the fold is float64, the rest in Real.
+template <class Real, bool Grad>
+__device__ __forceinline__ TransitPoint<Real> transit_point(double t, const double* theta) {
+    const double P = theta[0], T0 = theta[1];
+    const Real D = (Real)theta[2], delta = (Real)theta[3];
+    const Real q1 = (Real)theta[4], q2 = (Real)theta[5];
+    const double e = t - T0;
+    const double n = rint(e / P);
+    const Real tau = (Real)(e - n * P);
+    const Real x = (Real)2 * tau / D;
+    const Real s = (Real)1 - x * x;
+    const Real ks = (Real)kTransitKappa * s;
+    TransitPoint<Real> r{(Real)1, 0, 0, 0, 0, 0, 0};
+    if (ks < (Real)-kLogCut) return r;
+    // W = sigmoid(ks), softplus(ks) = max(ks, 0) + log1p(exp(-|ks|)), both from one exp
+    const Real ex = exp(-fabs(ks));
+    const Real W = ks >= 0 ? (Real)1 / ((Real)1 + ex) : ex / ((Real)1 + ex);
+    const Real sp = (ks > 0 ? ks : (Real)0) + log1p(ex);
+    const Real mu = sqrt(sp / (Real)kTransitKappa);
+    const Real sq1 = sqrt(q1);
+    const Real u1 = (Real)2 * sq1 * q2, u2 = sq1 * ((Real)1 - (Real)2 * q2);
+    const Real om = (Real)1 - mu;
+    const Real I = (Real)1 - u1 * om - u2 * om * om;
+    r.f = (Real)1 - delta * W * I;
+    if constexpr (Grad) {
+        const Real dWdx = (Real)kTransitKappa * W * ((Real)1 - W) * ((Real)-2 * x);
+        const Real dmudx = -W * x / mu;
+        const Real dIdx = (u1 + (Real)2 * u2 * om) * dmudx;
+        const Real dfdx = -delta * (dWdx * I + W * dIdx);
+        const Real two_d = (Real)2 / D;
+        r.dT0 = -dfdx * two_d;
+        r.dP = -dfdx * two_d * (Real)n;
+        r.dD = -dfdx * x / D;
+        r.ddelta = -W * I;
+        const Real dfdu1 = delta * W * om, dfdu2 = delta * W * om * om;
+        // du1/dq1 = q2 / sqrt(q1), du2/dq1 = (1 - 2 q2) / (2 sqrt(q1)),
+        // du1/dq2 = 2 sqrt(q1),     du2/dq2 = -2 sqrt(q1)
+        r.dq1 = (dfdu1 * q2 + dfdu2 * (Real)0.5 * ((Real)1 - (Real)2 * q2)) / sq1;
+        r.dq2 = (dfdu1 - dfdu2) * (Real)2 * sq1;
+    }
+    return r;
+}
+
+// Model m covers the points [pt_off[c], pt_off[c + 1]) of curve c = model_curve[m] and
+// writes its fluxes at flux[out_off[m] ..]. Grid: (x blocks over points, y = model).
+template <class Real>
+__global__ __launch_bounds__(256) void transit_fwd_kernel(const double* __restrict__ theta,
+                                                          const int32_t* __restrict__ model_curve,
+                                                          const int64_t* __restrict__ pt_off,
+                                                          const double* __restrict__ t,
+                                                          const int64_t* __restrict__ out_off,
+                                                          Real* __restrict__ flux) {
+    const int64_t m = blockIdx.y;
+    const int c = model_curve[m];
+    const int64_t a = pt_off[c], n = pt_off[c + 1] - a;
+    double th[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) th[k] = theta[6 * m + k];
+    Real* out = flux + out_off[m];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = transit_point<Real, false>(t[a + i], th).f;
+}
+
+// grad_theta[m][k] = sum_i grad_flux[out_off[m] + i] * df_i/dtheta_k, one workgroup per
+// model; float64 accumulation.
+template <class Real>
+__global__ __launch_bounds__(256) void transit_bwd_kernel(const double* __restrict__ theta,
+                                                          const int32_t* __restrict__ model_curve,
+                                                          const int64_t* __restrict__ pt_off,
+                                                          const double* __restrict__ t,
+                                                          const int64_t* __restrict__ out_off,
+                                                          const Real* __restrict__ grad_flux,
+                                                          double* __restrict__ grad_theta) {
+    __shared__ double red[6][256 / kWave];
+    const int64_t m = blockIdx.x;
+    const int c = model_curve[m];
+    const int64_t a = pt_off[c], n = pt_off[c + 1] - a;
+    double th[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) th[k] = theta[6 * m + k];
+    const Real* g = grad_flux + out_off[m];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const Real gi = g[i];
+        if (gi == (Real)0) continue;
+        const TransitPoint<Real> p = transit_point<Real, true>(t[a + i], th);
+        acc[0] += (double)(gi * p.dP);
+        acc[1] += (double)(gi * p.dT0);
+        acc[2] += (double)(gi * p.dD);
+        acc[3] += (double)(gi * p.ddelta);
+        acc[4] += (double)(gi * p.dq1);
+        acc[5] += (double)(gi * p.dq2);
+    }
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += red[threadIdx.x][w];
+        grad_theta[6 * m + threadIdx.x] = s;
+    }
+}
+
+}  // namespace gtls

@@ -27,6 +27,8 @@
  *   gtls_spectra_batched
  *       spectra() of src/gputls/stats.py:95-122 (SR, power_raw, detrended power, SDE)
  *       with the running median of src/gputls/helpers.py:91-106, per light curve.
+ *   gtls_transit_fwd / gtls_transit_bwd
+ *       no reference counterpart: the model of the gradient refinement of grid peaks.
  */
 #ifndef GTLS_AMD_H
 #define GTLS_AMD_H
@@ -143,6 +145,29 @@ int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off
                          const double* chi2, int64_t total_periods, int32_t kernel,
                          void* workspace, int64_t workspace_bytes, double* power_raw,
                          double* power, double* sde_raw, double* sde, int32_t* status);
+
+/*
+ * Smooth limb-darkened transit model for refining grid peaks by gradient descent. No
+ * counterpart in the reference (it refines on the host, src/gputls/stats.py:233-488,
+ * without a model fit); these two calls are the forward and backward of one
+ * torch.autograd.Function.
+ *
+ * Model m (theta[6 m .. 6 m + 5] = period, T0, duration, depth, q1, q2; float64) covers
+ * the points [pt_off[c], pt_off[c + 1]) of curve c = model_curve[m] (times t, float64)
+ * and owns the flux slots [out_off[m], out_off[m] + N_c). Fluxes and their gradients are
+ * float64 in GTLS_MODE_FP64 and float32 in GTLS_MODE_FP32C (the fold is float64 in both).
+ *
+ * gtls_transit_fwd writes the model flux of every point; max_points (the largest N_c) only
+ * sizes the launch; n_models <= 65535.
+ * gtls_transit_bwd writes grad_theta[6 m + k] = sum_i grad_flux[out_off[m] + i] *
+ * d flux_i / d theta_k, accumulated in float64.
+ */
+int gtls_transit_fwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
+                     const int32_t* model_curve, const int64_t* pt_off, const double* t,
+                     const int64_t* out_off, int64_t max_points, void* flux);
+int gtls_transit_bwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
+                     const int32_t* model_curve, const int64_t* pt_off, const double* t,
+                     const int64_t* out_off, const void* grad_flux, double* grad_theta);
 
 #ifdef __cplusplus
 }
