@@ -413,7 +413,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     for (int b = tid; b < B; b += nt) hist[b] = 0;
     if (tid < 4) counts[tid] = 0;
     __syncthreads();
-    constexpr int U = 4;   // samples in flight per thread
+    constexpr int U = 8;   // samples in flight per thread
     for (int64_t base = 0; base < n; base += (int64_t)U * nt) {
         double tv[U];
 #pragma unroll
@@ -490,7 +490,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     __syncthreads();
     // one pass: every sample's index into its window's range of `binned`
     {
-        constexpr int UW = 8;
+        constexpr int UW = 16;
         for (int64_t base = 0; base < n; base += (int64_t)UW * nt) {
             int bv[UW];
 #pragma unroll

@@ -24,7 +24,8 @@ namespace gtls {
 
 constexpr int kBlock = 256;
 constexpr int64_t kTmplTail = 32768;  // zeroed samples after the last template row
-constexpr int kScanPerThread = 8;  // gathered elements per thread per prefix-scan chunk
+constexpr int kScanPerThread = 11;  // gathered elements per thread per prefix-scan chunk
+static_assert(kBlock * kScanPerThread >= kWinCap, "one prefix-scan chunk per sort window");
 
 
 // Per trial period of a chunk: written by prep_kernel, read by the two later stages.
