@@ -165,6 +165,55 @@ __device__ __forceinline__ void gather_scan(const typename Pair<Real>::type* __r
     }
 }
 
+// gather_scan for one sort window of cnt <= kWinCap samples in float32 with a common
+// weight (X = u): each thread gathers kScanPerThread consecutive positions, sums them
+// serially, and one workgroup scan of the thread totals gives every position's running sum
+// of u (float64); u and D are staged in the window's LDS (su over the sample indices sidx,
+// sD over the phases) and copied out coalesced. All threads call it.
+__device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw, int64_t o0,
+                                                   int cnt, int32_t* sidx, double* sD,
+                                                   float* ub, double* D, double& carry,
+                                                   double* red_d) {
+    const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+    const int p0 = tid * kScanPerThread;
+    float* su = (float*)sidx;   // position p is read (as an index) before it is rewritten
+    float u[kScanPerThread];
+#pragma unroll
+    for (int e = 0; e < kScanPerThread; ++e) {
+        const int p = p0 + e;
+        u[e] = p < cnt ? uw[sidx[p]].x : 0.f;
+    }
+    double pre[kScanPerThread];
+    double run = 0.0;
+#pragma unroll
+    for (int e = 0; e < kScanPerThread; ++e) {
+        run += (double)u[e];
+        pre[e] = run;
+    }
+    const double incl = wave_inclusive_scan(run);
+    if (lane == kWave - 1) red_d[wave] = incl;
+    __syncthreads();
+    double before = carry + (incl - run), total = carry;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+        if (w < wave) before += red_d[w];
+        total += red_d[w];
+    }
+#pragma unroll
+    for (int e = 0; e < kScanPerThread; ++e) {
+        const int p = p0 + e;
+        if (p < cnt) {
+            sD[p] = before + pre[e];
+            su[p] = u[e];
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < cnt; j += kBlock) {
+        ub[o0 + j] = su[j];
+        D[o0 + j] = sD[j];
+    }
+    carry = total;
+}
+
 // Stage 1 + 2 of each period of the chunk, one workgroup per period (persistent):
 //   1. fold + stable phase sort (fold_sort.h)                  -> perm
 //   2. gather in phase order with the wrap-around extension    -> X, running sum D
@@ -234,6 +283,13 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
             const bool windowed = wg_fold_sort_windows(
                 p.t + pt0, n, P, p.negative_time != 0, bk, perm, hist, red_i, s_counts, s_phi,
                 s_idx, s_mids, s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
+                    if constexpr (sizeof(Real) == 4) {
+                        if (constw) {
+                            gather_scan_window((const float2*)uw, wstart, wcnt, s_idx, s_phi,
+                                               (float*)ub, D, carry, red_d);
+                            return;
+                        }
+                    }
                     gather_scan<Real>(uw, wstart, wcnt, [&](int64_t j) { return s_idx[j]; },
                                       constw, ub, wpl, D, carry, red_d);
                 });
@@ -397,7 +453,7 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
 // then from the other XCDs' queues once its own is drained. Every wavefront leaves after
 // it found all eight queues empty.
 template <class Real>
-__global__ __launch_bounds__(kBlock) void windows_kernel(SearchParams<Real> p) {
+__global__ __launch_bounds__(kBlock, 2) void windows_kernel(SearchParams<Real> p) {
     const int lane = lane_id();
     // One fetch per iteration and no early exits: the loop ends once all eight queues were
     // seen empty (each queue is visited until it is, in order x0, x0 + 1, ...).

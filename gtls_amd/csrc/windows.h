@@ -504,6 +504,17 @@ __device__ __forceinline__ ArgMin toep_width(const float* __restrict__ X,
     g_load(sweep(X, T, 0), 0, s0);
     for (int64_t st = 0; st < nsuper; ++st) {
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
+        // the window-mean prefix sums D[s + d - 1] and D[s - 1] (0 at s = 0) of the lane's
+        // 16 candidates (1024 st + 256 t + 16 (4 kk + v) + col), requested before the
+        // sweeps so that they arrive while the MFMAs run
+        double dlo[16], dhi[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int64_t m = 1024 * st + 256 * (e >> 2) + 16 * (4 * kk + (e & 3)) + col;
+            const int64_t s = (m < K ? m : 0) * q;
+            dhi[e] = D[s + d - 1];
+            dlo[e] = D[s > 0 ? s - 1 : 0];
+        }
         // A sums (and B sums with the weights); the first group of the next sweep is
         // loaded during this one
         if constexpr (ConstW) {
@@ -515,20 +526,13 @@ __device__ __forceinline__ ArgMin toep_width(const float* __restrict__ X,
         // the statistic of the lane's 16 candidates
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            double dlo[4], dhi[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t m = 1024 * st + 256 * t + 16 * (4 * kk + v) + col;
-                const int64_t s = (m < K ? m : 0) * q;
-                dhi[v] = D[s + d - 1];
-                dlo[v] = s ? D[s - 1] : 0.0;
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
+                const int e = 4 * t + v;
                 const int64_t m = 1024 * st + 256 * t + 16 * (4 * kk + v) + col;
                 if (m < K) {
                     const int32_t s = (int32_t)(m * q);
-                    const double mm = (dhi[v] - dlo[v]) * a.inv_d;
+                    const double mm = (dhi[e] - (s > 0 ? dlo[e] : 0.0)) * a.inv_d;
                     if (mm > a.depth_min) {
                         const float rs = (float)(mm * a.os2);
                         const float sa = acc[t][v];
