@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "gtls_amd.h"
 #include "search.h"
@@ -24,6 +25,14 @@ struct gtls_ctx {
     int toep_q = gtls::kToepQ;
     int64_t chunk_bytes = 0;            // budget of one chunk's gathered samples
     std::string err;
+    // kernel timing (gtls_set_timing): per period chunk, events before the prep kernel,
+    // between prep and windows, between windows and reduce, after reduce
+    int timing = 0;
+    std::vector<hipEvent_t> pool;       // events owned by the context
+    size_t used = 0;                    // events recorded since the last gtls_kernel_ms
+    ~gtls_ctx() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -257,17 +266,33 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.chi2 = chi2;
     p.best_k = best_k;
     p.best_s = best_s;
+    // with timing on, an event before and after each of the chunk's three kernels
+    auto mark = [&]() -> hipError_t {
+        if (!ctx->timing) return hipSuccess;
+        if (ctx->used == ctx->pool.size()) {
+            hipEvent_t e;
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ctx->pool.push_back(e);
+        }
+        return hipEventRecord(ctx->pool[ctx->used++], st);
+    };
     for (int64_t ci = 0; ci < L.n_chunks && b->total_periods > 0; ++ci) {
         p.p0 = ci * L.chunk;
         p.chunk = std::min<int64_t>(L.chunk, b->total_periods - p.p0);
         p.counters = counters + 16 * ci;
         const int64_t g1 = std::min<int64_t>(L.n_slots, p.chunk);
+        hipError_t te = mark();
         hipLaunchKernelGGL(prep_kernel<Real>, dim3((unsigned)g1), dim3(kBlock), 0, st, p);
+        if (te == hipSuccess) te = mark();
         const int64_t g2 = std::min<int64_t>(L.n_win, (p.chunk * L.kmax + 3) / 4);
         hipLaunchKernelGGL(windows_kernel<Real>, dim3((unsigned)std::max<int64_t>(1, g2)),
                            dim3(kBlock), 0, st, p);
+        if (te == hipSuccess) te = mark();
         hipLaunchKernelGGL(reduce_kernel<Real>, dim3((unsigned)((p.chunk + tpb - 1) / tpb)),
                            dim3(tpb), 0, st, p);
+        if (te == hipSuccess) te = mark();
+        if (te != hipSuccess) return hip_fail(ctx, te, "timing event");
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "search launch");
@@ -330,6 +355,35 @@ int gtls_set_stream(gtls_ctx* ctx, void* stream) {
 }
 
 const char* gtls_last_error(const gtls_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int gtls_set_timing(gtls_ctx* ctx, int32_t on) {
+    if (!ctx) return GTLS_E_SHAPE;
+    ctx->timing = on != 0;
+    ctx->used = 0;
+    return GTLS_OK;
+}
+
+int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (!ms) return fail(ctx, GTLS_E_SHAPE, "NULL pointer");
+    ms[0] = ms[1] = ms[2] = 0.0;
+    const size_t n = ctx->used / 4 * 4;
+    DeviceGuard guard(ctx->device);
+    if (n > 0) {
+        const hipError_t e = hipEventSynchronize(ctx->pool[n - 1]);
+        if (e != hipSuccess) return hip_fail(ctx, e, "timing sync");
+    }
+    for (size_t i = 0; i < n; i += 4) {
+        for (int k = 0; k < 3; ++k) {
+            float t = 0.f;
+            const hipError_t e = hipEventElapsedTime(&t, ctx->pool[i + k], ctx->pool[i + k + 1]);
+            if (e != hipSuccess) return hip_fail(ctx, e, "timing read");
+            ms[k] += t;
+        }
+    }
+    ctx->used = 0;
+    return (int64_t)(n / 4);
+}
 
 int32_t gtls_search_slots(const gtls_ctx* ctx) { return ctx ? ctx->prep_fp32 : 0; }
 

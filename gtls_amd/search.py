@@ -86,6 +86,26 @@ def _ptr(x):
     return ct.c_void_p(x.data_ptr())
 
 
+def set_kernel_timing(device=0, on=True):
+    """Record HIP events around the search kernels of later searches on torch's current
+    stream of `device` (gtls_set_timing)."""
+    dev = _device(device)
+    ctx = _ctx(dev)
+    _ffi.check(ctx.handle, _ffi.load().gtls_set_timing(ctx.handle, 1 if on else 0))
+
+
+def kernel_ms(device=0):
+    """(prep, windows, reduce) milliseconds of the search kernels recorded since the last
+    call, and the number of period chunks (gtls_kernel_ms; waits for them to finish)."""
+    dev = _device(device)
+    ctx = _ctx(dev)
+    ms = (ct.c_double * 3)()
+    n = _ffi.load().gtls_kernel_ms(ctx.handle, ms)
+    if n < 0:
+        _ffi.check(ctx.handle, int(n))
+    return (ms[0], ms[1], ms[2]), int(n)
+
+
 _WORKSPACES = {}
 
 

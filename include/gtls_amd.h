@@ -29,6 +29,8 @@
  *       with the running median of src/gputls/helpers.py:91-106, per light curve.
  *   gtls_transit_fwd / gtls_transit_bwd
  *       no reference counterpart: the model of the gradient refinement of grid peaks.
+ *   gtls_set_timing / gtls_kernel_ms
+ *       no reference counterpart: per-kernel timing of the search chain.
  */
 #ifndef GTLS_AMD_H
 #define GTLS_AMD_H
@@ -145,6 +147,18 @@ int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off
                          const double* chi2, int64_t total_periods, int32_t kernel,
                          void* workspace, int64_t workspace_bytes, double* power_raw,
                          double* power, double* sde_raw, double* sde, int32_t* status);
+
+/*
+ * Kernel timing of the search (no reference counterpart; for measuring the chain's kernels
+ * from the calling program). With gtls_set_timing(ctx, 1) every later search call records
+ * HIP events on the context's stream around each period chunk's prep, windows and reduce
+ * kernels (the context keeps and reuses the events; gtls_set_timing also discards earlier
+ * records). gtls_kernel_ms waits for the last recorded event, writes the summed
+ * milliseconds of the prep, windows and reduce kernels since the previous call into
+ * ms[0..2], clears the records and returns the number of chunks (or a negative error).
+ */
+int gtls_set_timing(gtls_ctx* ctx, int32_t on);
+int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms);
 
 /*
  * Smooth limb-darkened transit model for refining grid peaks by gradient descent. No
