@@ -226,6 +226,23 @@ class DeviceBatch:
         return dict(power=power, power_raw=power_raw, SDE=sde, SDE_raw=sde_raw,
                     device=status == 0)
 
+    def capture(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
+        """A HIP graph (torch.cuda.CUDAGraph) of search(): replay() re-runs every size
+        group's search call into chi2 / best_k / best_s with no host work. The search runs
+        once on the graph's stream first, so that its library context and workspace exist
+        before the capture; every kernel of the call is captured (no memset nodes)."""
+        import torch
+        s = torch.cuda.Stream(self.dev)
+        s.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(s):
+            self.search(mode, skip_point, depth_min)
+        torch.cuda.current_stream(self.dev).wait_stream(s)
+        torch.cuda.synchronize(self.dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self.search(mode, skip_point, depth_min)
+        return g
+
     def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
         """Enqueue the batched search on torch's current stream (no synchronisation): one
         gtls_search_batched call per size group, each writing its slice of the outputs."""
