@@ -26,7 +26,7 @@ EXPORTED = (
     "gtls_search_workspace_bytes", "gtls_search_batched", "gtls_search_workspace_bytes_1",
     "gtls_search", "gtls_fold_sort_workspace_bytes", "gtls_fold_sort",
     "gtls_spectra_workspace_bytes", "gtls_spectra_batched", "gtls_transit_fwd",
-    "gtls_transit_bwd", "gtls_set_timing", "gtls_kernel_ms",
+    "gtls_transit_bwd", "gtls_set_timing", "gtls_kernel_ms", "gtls_set_tuning",
 )
 
 P = ct.c_void_p
@@ -78,6 +78,7 @@ def _declare(lib):
         "gtls_transit_bwd": (ct.c_int, [P, I32, I64, P, P, P, P, P, P, P]),
         "gtls_set_timing": (ct.c_int, [P, I32]),
         "gtls_kernel_ms": (I64, [P, ct.POINTER(F64)]),
+        "gtls_set_tuning": (ct.c_int, [P, I32, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -356,6 +356,15 @@ int gtls_set_stream(gtls_ctx* ctx, void* stream) {
 
 const char* gtls_last_error(const gtls_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
+int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (toep_q < 0 || toep_q > kToepQ) return fail(ctx, GTLS_E_SHAPE, "toep_q out of [0, 24]");
+    if (chunk_mb < 0) return fail(ctx, GTLS_E_SHAPE, "chunk_mb < 0");
+    ctx->toep_q = toep_q;
+    if (chunk_mb > 0) ctx->chunk_bytes = chunk_mb << 20;
+    return GTLS_OK;
+}
+
 int gtls_set_timing(gtls_ctx* ctx, int32_t on) {
     if (!ctx) return GTLS_E_SHAPE;
     ctx->timing = on != 0;

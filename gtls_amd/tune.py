@@ -1,0 +1,65 @@
+"""Autotuning of the float32 batched search on the device it runs on.
+
+The tunables are those of gtls_set_tuning: the stride below which a width takes the
+Toeplitz matrix-core tiles instead of the diagonal-sum tiles, and the buffer budget of one
+period chunk (which sets how many kernel tails a call pays). `autotune` times one search
+call of a sample batch (curves spread over the batch's sizes) under each candidate setting
+with the library's per-kernel HIP events, and applies the fastest to the context of the
+device's current stream; later searches from this process on that stream use it.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _ffi
+from . import constants as C
+from .search import _ctx, _device, kernel_ms, set_kernel_timing
+
+TOEP_Q = (12, 16, 24)
+CHUNK_MB = (8 << 10, 32 << 10)
+
+
+def set_tuning(device=0, toep_q=24, chunk_mb=0):
+    """Apply a setting to the context of `device`'s current torch stream."""
+    dev = _device(device)
+    ctx = _ctx(dev)
+    _ffi.check(ctx.handle, _ffi.load().gtls_set_tuning(ctx.handle, int(toep_q), int(chunk_mb)))
+
+
+def sample_batch(db, n=32):
+    """A DeviceBatch of up to n curves of db's plan, spread over its curve sizes."""
+    from .batch import BatchPlan, DeviceBatch
+    bp = db.bp
+    order = np.argsort([p.n for p in bp.plans], kind="stable")
+    pick = order[np.linspace(0, len(order) - 1, min(n, len(order))).astype(int)]
+    return DeviceBatch(BatchPlan([bp.plans[i] for i in sorted(set(pick.tolist()))]), db.dev.index)
+
+
+def autotune(db, skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, n_sample=32, repeats=2):
+    """Times the sample's search under every (toep_q, chunk_mb) candidate (the best of
+    `repeats` calls each, after one untimed call) and applies the fastest. Returns
+    (best setting, {setting: milliseconds})."""
+    import torch
+    sb = sample_batch(db, n_sample)
+    dev = db.dev.index
+    times = {}
+    set_kernel_timing(dev, True)
+    try:
+        for q in TOEP_Q:
+            for mb in CHUNK_MB:
+                set_tuning(dev, q, mb)
+                sb.search("fp32c", skip_point, depth_min)
+                torch.cuda.synchronize(db.dev)
+                kernel_ms(dev)
+                best = np.inf
+                for _ in range(repeats):
+                    sb.search("fp32c", skip_point, depth_min)
+                    torch.cuda.synchronize(db.dev)
+                    (pm, wm, rm), _ = kernel_ms(dev)
+                    best = min(best, pm + wm + rm)
+                times[(q, mb)] = best
+    finally:
+        set_kernel_timing(dev, False)
+    choice = min(times, key=times.get)
+    set_tuning(dev, *choice)
+    return choice, times

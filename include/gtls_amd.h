@@ -29,8 +29,8 @@
  *       with the running median of src/gputls/helpers.py:91-106, per light curve.
  *   gtls_transit_fwd / gtls_transit_bwd
  *       no reference counterpart: the model of the gradient refinement of grid peaks.
- *   gtls_set_timing / gtls_kernel_ms
- *       no reference counterpart: per-kernel timing of the search chain.
+ *   gtls_set_timing / gtls_kernel_ms / gtls_set_tuning
+ *       no reference counterpart: per-kernel timing and tuning of the search chain.
  */
 #ifndef GTLS_AMD_H
 #define GTLS_AMD_H
@@ -158,6 +158,14 @@ int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off
  * ms[0..2], clears the records and returns the number of chunks (or a negative error).
  */
 int gtls_set_timing(gtls_ctx* ctx, int32_t on);
+
+/*
+ * Tuning of the float32 search (no reference counterpart; set by gtls_amd.tune): strides q
+ * below toep_q (0 .. 24) take the Toeplitz matrix-core tiles, larger ones the
+ * diagonal-sum tiles; chunk_mb (0 keeps the current value) bounds the per-period buffers
+ * of one period chunk. Applies to later calls on the context.
+ */
+int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb);
 int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms);
 
 /*

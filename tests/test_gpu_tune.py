@@ -1,0 +1,32 @@
+"""Autotuning of the float32 search: every candidate setting is timed, the fastest is
+applied, and a search under any setting agrees with the default one to the float32 bar."""
+import numpy as np
+import pytest
+import torch
+
+from gtls_amd import batch as B
+from gtls_amd import synthetic
+from gtls_amd import tune
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_picks_a_candidate_and_keeps_results(gpu):
+    curves = synthetic.batch(24, seed=5, n_min=2000, n_max=12000)
+    db = B.DeviceBatch(B.BatchPlan(B.build_plans(curves, {})), 0)
+    c0 = db.search("fp32c", 8)[0].clone()
+    torch.cuda.synchronize()
+    try:
+        choice, times = tune.autotune(db, n_sample=12, repeats=1)
+        assert len(times) == len(tune.TOEP_Q) * len(tune.CHUNK_MB)
+        assert all(np.isfinite(v) and v > 0 for v in times.values())
+        assert choice in times and times[choice] == min(times.values())
+        for q in tune.TOEP_Q:
+            tune.set_tuning(0, q, 0)
+            c = db.search("fp32c", 8)[0]
+            torch.cuda.synchronize()
+            ok = torch.isfinite(c0)
+            rel = ((c - c0).abs() / c0.abs())[ok]
+            assert float(rel.max()) < 2e-6, q
+    finally:
+        tune.set_tuning(0, 24, 32 << 10)
