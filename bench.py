@@ -112,11 +112,14 @@ def refine_leg(args, plans, batch0, skip):
     RF.Refiner(mb, thetas, spans).run(2)          # warm-up (kernels, allocator)
     ref = RF.Refiner(mb, thetas, spans)
     torch.cuda.synchronize()
+    RF.set_timing(True)
     ts = time.perf_counter()
     first, last = ref.run(args.refine_steps,
                           progress=lambda i: log("refine: step %d done" % i))
     torch.cuda.synchronize()
     dt = time.perf_counter() - ts
+    kt = RF.timing_ms()
+    RF.set_timing(False)
     ratio = (last / first).cpu().numpy()
     slots = mb.slots
     out = {"curves": len(curves), "models": int(mb.n_models), "steps": args.refine_steps,
@@ -126,6 +129,15 @@ def refine_leg(args, plans, batch0, skip):
            "curves_per_hour": round(len(curves) / dt * 3600.0, 1),
            "chi2_ratio_median": float(np.median(ratio)),
            "chi2_improved_models": "%d/%d" % (int(np.sum(ratio < 1)), len(ratio))}
+    # the two kernels against the HBM roofline: per point and call, 8 B of time and 4 B of
+    # flux (forward: written; backward: the upstream gradient, read)
+    for k, name in (("fwd", "transit_fwd_kernel"), ("bwd", "transit_bwd_kernel")):
+        ms, calls = kt.get(k, (0.0, 0))
+        if calls:
+            gbs = 12.0 * slots * calls / (ms * 1e-3) / 1e9
+            out[name] = {"ms_per_call": round(ms / calls, 4), "calls": calls,
+                         "algorithmic_GBps": round(gbs, 1),
+                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
     log("refine: %d models, %d steps in %.2f s (%.2f ms/step), median chi2 ratio %.4f" % (
         mb.n_models, args.refine_steps, dt, out["ms_per_step"], out["chi2_ratio_median"]))
     return out

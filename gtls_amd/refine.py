@@ -96,6 +96,42 @@ class ModelBatch:
         return int(self.y.numel())
 
 
+# optional HIP-event timing of the two kernels: {"fwd": [(start, end), ...], "bwd": [...]}
+_TIMING = None
+
+
+def set_timing(on=True):
+    """Record torch (HIP) events around every gtls_transit_fwd / _bwd call from now on."""
+    global _TIMING
+    _TIMING = {"fwd": [], "bwd": []} if on else None
+
+
+def timing_ms():
+    """Summed milliseconds and call counts of the recorded forward and backward kernels
+    (waits for them), then clears the records."""
+    torch = _torch()
+    out = {}
+    if _TIMING is None:
+        return out
+    torch.cuda.synchronize()
+    for k, evs in _TIMING.items():
+        out[k] = (sum(a.elapsed_time(b) for a, b in evs), len(evs))
+        evs.clear()
+    return out
+
+
+def _timed(kind, fn):
+    if _TIMING is None:
+        return fn()
+    torch = _torch()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    r = fn()
+    b.record()
+    _TIMING[kind].append((a, b))
+    return r
+
+
 def _flux_dtype(mode):
     torch = _torch()
     return torch.float64 if mode == "fp64" else torch.float32
@@ -109,10 +145,11 @@ def transit_fwd(mb: ModelBatch, theta, mode="fp32c"):
         raise ValueError("theta must be a float64 [n_models, 6] tensor on the batch's device")
     flux = torch.empty(mb.slots, dtype=_flux_dtype(mode), device=mb.dev)
     ctx = _ctx(mb.dev)
-    _ffi.check(ctx.handle, _ffi.load().gtls_transit_fwd(
+    code = _timed("fwd", lambda: _ffi.load().gtls_transit_fwd(
         ctx.handle, MODES[mode], mb.n_models, _ffi.ptr(theta), _ffi.ptr(mb.model_curve),
         _ffi.ptr(mb.pt_off), _ffi.ptr(mb.t), _ffi.ptr(mb.out_off), mb.max_points,
         _ffi.ptr(flux)))
+    _ffi.check(ctx.handle, code)
     return flux
 
 
@@ -125,10 +162,11 @@ def transit_bwd(mb: ModelBatch, theta, grad_flux, mode="fp32c"):
         raise ValueError("grad_flux has %d slots, the batch %d" % (g.numel(), mb.slots))
     out = torch.empty((mb.n_models, 6), dtype=torch.float64, device=mb.dev)
     ctx = _ctx(mb.dev)
-    _ffi.check(ctx.handle, _ffi.load().gtls_transit_bwd(
+    code = _timed("bwd", lambda: _ffi.load().gtls_transit_bwd(
         ctx.handle, MODES[mode], mb.n_models, _ffi.ptr(theta), _ffi.ptr(mb.model_curve),
         _ffi.ptr(mb.pt_off), _ffi.ptr(mb.t), _ffi.ptr(mb.out_off), _ffi.ptr(g),
         _ffi.ptr(out)))
+    _ffi.check(ctx.handle, code)
     return out
 
 
