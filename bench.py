@@ -86,7 +86,8 @@ def profiled_traffic(args):
 def refine_leg(args, plans, batch0, skip):
     """Gradient refinement of grid peaks: the 8 highest power peaks of each of the first
     `refine_curves` curves of batch 0 seed a smooth limb-darkened transit model each, fitted
-    by Adam (`refine_steps` steps, float32 model on the HIP kernels)."""
+    by Adam (`refine_steps` steps, float32 model; loss and gradient from the fused HIP
+    kernel)."""
     import torch
     from gtls_amd import refine as RF
     bp, db = batch0
@@ -129,12 +130,14 @@ def refine_leg(args, plans, batch0, skip):
            "curves_per_hour": round(len(curves) / dt * 3600.0, 1),
            "chi2_ratio_median": float(np.median(ratio)),
            "chi2_improved_models": "%d/%d" % (int(np.sum(ratio < 1)), len(ratio))}
-    # the two kernels against the HBM roofline: per point and call, 8 B of time and 4 B of
+    # the kernels against the HBM roofline, bytes per model point and call: the fused loss
+    # kernel reads time, flux and weight (24 B); the flux kernels 8 B of time and 4 B of
     # flux (forward: written; backward: the upstream gradient, read)
-    for k, name in (("fwd", "transit_fwd_kernel"), ("bwd", "transit_bwd_kernel")):
+    for k, name, per in (("chi2", "transit_chi2_kernel", 24.0),
+                         ("fwd", "transit_fwd_kernel", 12.0), ("bwd", "transit_bwd_kernel", 12.0)):
         ms, calls = kt.get(k, (0.0, 0))
         if calls:
-            gbs = 12.0 * slots * calls / (ms * 1e-3) / 1e9
+            gbs = per * slots * calls / (ms * 1e-3) / 1e9
             out[name] = {"ms_per_call": round(ms / calls, 4), "calls": calls,
                          "algorithmic_GBps": round(gbs, 1),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}

@@ -26,7 +26,7 @@ EXPORTED = (
     "gtls_search_workspace_bytes", "gtls_search_batched", "gtls_search_workspace_bytes_1",
     "gtls_search", "gtls_fold_sort_workspace_bytes", "gtls_fold_sort",
     "gtls_spectra_workspace_bytes", "gtls_spectra_batched", "gtls_transit_fwd",
-    "gtls_transit_bwd", "gtls_set_timing", "gtls_kernel_ms", "gtls_set_tuning",
+    "gtls_transit_bwd", "gtls_transit_chi2", "gtls_set_timing", "gtls_kernel_ms", "gtls_set_tuning",
 )
 
 P = ct.c_void_p
@@ -76,6 +76,7 @@ def _declare(lib):
         "gtls_spectra_batched": (ct.c_int, [P, I32, P, P, I64, I32, P, I64, P, P, P, P, P]),
         "gtls_transit_fwd": (ct.c_int, [P, I32, I64, P, P, P, P, P, I64, P]),
         "gtls_transit_bwd": (ct.c_int, [P, I32, I64, P, P, P, P, P, P, P]),
+        "gtls_transit_chi2": (ct.c_int, [P, I32, I64, P, P, P, P, P, P, P, P]),
         "gtls_set_timing": (ct.c_int, [P, I32]),
         "gtls_kernel_ms": (I64, [P, ct.POINTER(F64)]),
         "gtls_set_tuning": (ct.c_int, [P, I32, I64]),

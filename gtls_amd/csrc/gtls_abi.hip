@@ -620,4 +620,37 @@ int gtls_transit_bwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double
     return GTLS_OK;
 }
 
+int gtls_transit_chi2(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
+                      const int32_t* model_curve, const int64_t* pt_off, const double* t,
+                      const double* y, const double* w, double* chi2, double* grad_theta) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C)
+        return fail(ctx, GTLS_E_MODE, "unknown mode");
+    if (n_models < 0 || n_models > (1LL << 31) - 1)
+        return fail(ctx, GTLS_E_SHAPE, "bad transit model count");
+    if (n_models == 0) return GTLS_OK;
+    if (!theta || !model_curve || !pt_off || !t || !y || !w || !chi2)
+        return fail(ctx, GTLS_E_SHAPE, "null transit model argument");
+    DeviceGuard guard(ctx->device);
+    const dim3 grid((unsigned)n_models), block(256);
+    if (mode == GTLS_MODE_FP64) {
+        if (grad_theta)
+            hipLaunchKernelGGL((transit_chi2_kernel<double, true>), grid, block, 0, ctx->stream,
+                               theta, model_curve, pt_off, t, y, w, chi2, grad_theta);
+        else
+            hipLaunchKernelGGL((transit_chi2_kernel<double, false>), grid, block, 0, ctx->stream,
+                               theta, model_curve, pt_off, t, y, w, chi2, grad_theta);
+    } else {
+        if (grad_theta)
+            hipLaunchKernelGGL((transit_chi2_kernel<float, true>), grid, block, 0, ctx->stream,
+                               theta, model_curve, pt_off, t, y, w, chi2, grad_theta);
+        else
+            hipLaunchKernelGGL((transit_chi2_kernel<float, false>), grid, block, 0, ctx->stream,
+                               theta, model_curve, pt_off, t, y, w, chi2, grad_theta);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "transit chi2 launch");
+    return GTLS_OK;
+}
+
 }  // extern "C"

@@ -5,7 +5,8 @@
 // Parameters of one model (theta, 6 float64 values): period P, mid-transit time T0,
 // duration D (days), depth delta, and Kipping's limb-darkening parameters q1, q2 in (0, 1)
 // (u1 = 2 sqrt(q1) q2, u2 = sqrt(q1) (1 - 2 q2)). For a time t:
-//     e = t - T0,  n = rint(e / P),  tau = e - n P             (float64: the fold)
+//     e = t - T0,  n = rint(e / P),  tau = e - n P             (float64: the fold; e / P
+//                                                               as e times 1 / P)
 //     x = 2 tau / D                                            (chord coordinate)
 //     s = 1 - x^2,  W = sigmoid(kappa s),  mu = sqrt(softplus(kappa s) / kappa)
 //     I = 1 - u1 (1 - mu) - u2 (1 - mu)^2                      (quadratic law)
@@ -35,15 +36,18 @@ struct TransitPoint {
     Real dP, dT0, dD, ddelta, dq1, dq2;  // its partial derivatives
 };
 
-// Model flux of one point, and (when Grad) its six partial derivatives. theta as above;
-// the fold is float64, the rest in Real.
+// Model flux of one point, and (when Grad) its six partial derivatives. theta as above,
+// inv_p = 1 / P; the fold is float64, the rest in Real. The product e inv_p can differ from
+// the quotient e / P in the last place, which moves n only for points half a period from
+// T0, where the model is 1 with zero derivatives for any duration below P / 2.
 template <class Real, bool Grad>
-__device__ __forceinline__ TransitPoint<Real> transit_point(double t, const double* theta) {
+__device__ __forceinline__ TransitPoint<Real> transit_point(double t, const double* theta,
+                                                            double inv_p) {
     const double P = theta[0], T0 = theta[1];
     const Real D = (Real)theta[2], delta = (Real)theta[3];
     const Real q1 = (Real)theta[4], q2 = (Real)theta[5];
     const double e = t - T0;
-    const double n = rint(e / P);
+    const double n = rint(e * inv_p);
     const Real tau = (Real)(e - n * P);
     const Real x = (Real)2 * tau / D;
     const Real s = (Real)1 - x * x;
@@ -94,14 +98,16 @@ __global__ __launch_bounds__(256) void transit_fwd_kernel(const double* __restri
     double th[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) th[k] = theta[6 * m + k];
+    const double inv_p = 1.0 / th[0];
     Real* out = flux + out_off[m];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = transit_point<Real, false>(t[a + i], th).f;
+        out[i] = transit_point<Real, false>(t[a + i], th, inv_p).f;
 }
 
 // grad_theta[m][k] = sum_i grad_flux[out_off[m] + i] * df_i/dtheta_k, one workgroup per
-// model; float64 accumulation.
+// model; float64 accumulation. Each thread has kBwdUnroll points' loads in flight.
+constexpr int kBwdUnroll = 4;
 template <class Real>
 __global__ __launch_bounds__(256) void transit_bwd_kernel(const double* __restrict__ theta,
                                                           const int32_t* __restrict__ model_curve,
@@ -117,18 +123,32 @@ __global__ __launch_bounds__(256) void transit_bwd_kernel(const double* __restri
     double th[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) th[k] = theta[6 * m + k];
+    const double inv_p = 1.0 / th[0];
     const Real* g = grad_flux + out_off[m];
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const Real gi = g[i];
-        if (gi == (Real)0) continue;
-        const TransitPoint<Real> p = transit_point<Real, true>(t[a + i], th);
-        acc[0] += (double)(gi * p.dP);
-        acc[1] += (double)(gi * p.dT0);
-        acc[2] += (double)(gi * p.dD);
-        acc[3] += (double)(gi * p.ddelta);
-        acc[4] += (double)(gi * p.dq1);
-        acc[5] += (double)(gi * p.dq2);
+    const int64_t nt = blockDim.x;
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += kBwdUnroll * nt) {
+        Real gv[kBwdUnroll];
+        double tv[kBwdUnroll];
+#pragma unroll
+        for (int u = 0; u < kBwdUnroll; ++u) {
+            const int64_t i = i0 + u * nt;
+            const bool in = i < n;
+            gv[u] = in ? g[in ? i : 0] : (Real)0;
+            tv[u] = in ? t[a + (in ? i : 0)] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBwdUnroll; ++u) {
+            const Real gi = gv[u];
+            if (gi == (Real)0) continue;
+            const TransitPoint<Real> p = transit_point<Real, true>(tv[u], th, inv_p);
+            acc[0] += (double)(gi * p.dP);
+            acc[1] += (double)(gi * p.dT0);
+            acc[2] += (double)(gi * p.dD);
+            acc[3] += (double)(gi * p.ddelta);
+            acc[4] += (double)(gi * p.dq1);
+            acc[5] += (double)(gi * p.dq2);
+        }
     }
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
@@ -141,6 +161,77 @@ __global__ __launch_bounds__(256) void transit_bwd_kernel(const double* __restri
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += red[threadIdx.x][w];
         grad_theta[6 * m + threadIdx.x] = s;
+    }
+}
+
+// The fit's loss and its gradient in one pass, without flux slots: for model m over the
+// points of curve c = model_curve[m] (times t, fluxes y, weights w, all float64),
+//     chi2[m] = sum_i w_i (y_i - f_i)^2,
+//     grad_theta[6 m + k] = sum_i -2 w_i (y_i - f_i) df_i/dtheta_k   (when Grad).
+// One workgroup per model; the residuals and sums are float64 (f in Real). Each thread has
+// kBwdUnroll points' loads in flight.
+template <class Real, bool Grad>
+__global__ __launch_bounds__(256) void transit_chi2_kernel(const double* __restrict__ theta,
+                                                           const int32_t* __restrict__ model_curve,
+                                                           const int64_t* __restrict__ pt_off,
+                                                           const double* __restrict__ t,
+                                                           const double* __restrict__ y,
+                                                           const double* __restrict__ w,
+                                                           double* __restrict__ chi2,
+                                                           double* __restrict__ grad_theta) {
+    constexpr int kSums = Grad ? 7 : 1;
+    __shared__ double red[kSums][256 / kWave];
+    const int64_t m = blockIdx.x;
+    const int c = model_curve[m];
+    const int64_t a = pt_off[c], n = pt_off[c + 1] - a;
+    double th[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) th[k] = theta[6 * m + k];
+    const double inv_p = 1.0 / th[0];
+    double acc[kSums];
+#pragma unroll
+    for (int k = 0; k < kSums; ++k) acc[k] = 0.0;
+    const int64_t nt = blockDim.x;
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += kBwdUnroll * nt) {
+        double tv[kBwdUnroll], yv[kBwdUnroll], wv[kBwdUnroll];
+#pragma unroll
+        for (int u = 0; u < kBwdUnroll; ++u) {
+            const int64_t i = i0 + u * nt;
+            const bool in = i < n;
+            const int64_t j = a + (in ? i : 0);
+            tv[u] = in ? t[j] : 0.0;
+            yv[u] = in ? y[j] : 1.0;
+            wv[u] = in ? w[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBwdUnroll; ++u) {
+            const TransitPoint<Real> p = transit_point<Real, Grad>(tv[u], th, inv_p);
+            const double r = yv[u] - (double)p.f;
+            const double wr = wv[u] * r;
+            acc[0] = fma(wr, r, acc[0]);
+            if constexpr (Grad) {
+                const double g = -2.0 * wr;
+                acc[1] = fma(g, (double)p.dP, acc[1]);
+                acc[2] = fma(g, (double)p.dT0, acc[2]);
+                acc[3] = fma(g, (double)p.dD, acc[3]);
+                acc[4] = fma(g, (double)p.ddelta, acc[4]);
+                acc[5] = fma(g, (double)p.dq1, acc[5]);
+                acc[6] = fma(g, (double)p.dq2, acc[6]);
+            }
+        }
+    }
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < kSums; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kSums) {
+        double s = 0.0;
+        for (int v = 0; v < (int)(blockDim.x / kWave); ++v) s += red[threadIdx.x][v];
+        if (threadIdx.x == 0) chi2[m] = s;
+        else grad_theta[6 * m + threadIdx.x - 1] = s;
     }
 }
 

@@ -69,6 +69,8 @@ class ModelBatch:
     w: object            # float64 [total slots]: its weight 1 / dy^2
     max_points: int
     n_models: int
+    y_pt: object = None  # float64 [total points]: flux per point (like t)
+    w_pt: object = None  # float64 [total points]: weight per point
 
     @classmethod
     def build(cls, curves, model_curve, device=0):
@@ -86,24 +88,29 @@ class ModelBatch:
         y = np.concatenate([yc[c] for c in model_curve]) if len(model_curve) else np.zeros(0)
         w = np.concatenate([wc[c] for c in model_curve]) if len(model_curve) else np.zeros(0)
         f64 = dict(dtype=torch.float64, device=dev)
+        y_pt = np.concatenate(yc) if yc else np.zeros(0)
+        w_pt = np.concatenate(wc) if wc else np.zeros(0)
         return cls(dev, torch.as_tensor(t, **f64), torch.as_tensor(pt_off, device=dev),
                    torch.as_tensor(model_curve, device=dev), torch.as_tensor(out_off, device=dev),
                    torch.as_tensor(y, **f64),
-                   torch.as_tensor(w, **f64), int(n.max(initial=0)), len(model_curve))
+                   torch.as_tensor(w, **f64), int(n.max(initial=0)), len(model_curve),
+                   torch.as_tensor(y_pt, **f64), torch.as_tensor(w_pt, **f64))
 
     @property
     def slots(self):
         return int(self.y.numel())
 
 
-# optional HIP-event timing of the two kernels: {"fwd": [(start, end), ...], "bwd": [...]}
+# optional HIP-event timing of the kernels: {"fwd": [(start, end), ...], "bwd": [...],
+# "chi2": [...]}
 _TIMING = None
 
 
 def set_timing(on=True):
-    """Record torch (HIP) events around every gtls_transit_fwd / _bwd call from now on."""
+    """Record torch (HIP) events around every gtls_transit_fwd / _bwd / _chi2 call from now
+    on."""
     global _TIMING
-    _TIMING = {"fwd": [], "bwd": []} if on else None
+    _TIMING = {"fwd": [], "bwd": [], "chi2": []} if on else None
 
 
 def timing_ms():
@@ -170,6 +177,58 @@ def transit_bwd(mb: ModelBatch, theta, grad_flux, mode="fp32c"):
     return out
 
 
+def transit_chi2(mb: ModelBatch, theta, mode="fp32c", grad=True):
+    """Per-model chi2 = sum_i w_i (y_i - f_i)^2 over the model's curve (float64 [n_models])
+    and, when grad, its gradient with respect to theta (float64 [n_models, 6]; else None),
+    in one fused pass (gtls_transit_chi2) that writes no flux slots."""
+    torch = _torch()
+    theta = theta.detach().to(torch.float64).contiguous()
+    if theta.shape != (mb.n_models, 6) or theta.device != mb.dev:
+        raise ValueError("theta must be a float64 [n_models, 6] tensor on the batch's device")
+    chi2 = torch.empty(mb.n_models, dtype=torch.float64, device=mb.dev)
+    g = torch.empty((mb.n_models, 6), dtype=torch.float64, device=mb.dev) if grad else None
+    ctx = _ctx(mb.dev)
+    code = _timed("chi2", lambda: _ffi.load().gtls_transit_chi2(
+        ctx.handle, MODES[mode], mb.n_models, _ffi.ptr(theta), _ffi.ptr(mb.model_curve),
+        _ffi.ptr(mb.pt_off), _ffi.ptr(mb.t), _ffi.ptr(mb.y_pt), _ffi.ptr(mb.w_pt),
+        _ffi.ptr(chi2), _ffi.ptr(g) if grad else None))
+    _ffi.check(ctx.handle, code)
+    return chi2, g
+
+
+def _chi2_function():
+    torch = _torch()
+
+    class TransitChi2(torch.autograd.Function):
+        """Per-model chi2 of the model against its curve; the gradient comes from the same
+        fused pass as the value."""
+
+        @staticmethod
+        def forward(ctx, theta, mb, mode):
+            chi2, g = transit_chi2(mb, theta, mode, grad=ctx.needs_input_grad[0])
+            ctx.save_for_backward(g if g is not None else chi2)
+            return chi2
+
+        @staticmethod
+        def backward(ctx, grad_chi2):
+            (g,) = ctx.saved_tensors
+            return (grad_chi2[:, None] * g).to(grad_chi2.dtype), None, None
+
+    return TransitChi2
+
+
+_TC = None
+
+
+def chi2_fused(theta, mb: ModelBatch, mode="fp32c"):
+    """Differentiable per-model chi2 (the fused kernel): the same value as
+    chi2_per_model(transit_flux(theta, mb, mode), mb)."""
+    global _TC
+    if _TC is None:
+        _TC = _chi2_function()
+    return _TC.apply(theta, mb, mode)
+
+
 def _function():
     torch = _torch()
 
@@ -221,12 +280,15 @@ class Refiner:
         P = P0 + sP zP, T0 = T0_0 + sT zT, D = D0 exp(zD / 4), depth = d0 exp(zd / 2),
         q1 = sigmoid(z1), q2 = sigmoid(z2),
     with sP = D0 P0 / (4 span) (a quarter duration of drift over the baseline) and
-    sT = D0 / 4."""
+    sT = D0 / 4. With fused (the default) the loss and its gradient come from one kernel
+    pass per step (gtls_transit_chi2); otherwise from the flux kernels and torch's
+    segmented sum."""
 
-    def __init__(self, mb: ModelBatch, theta0, spans, mode="fp32c", lr=0.05):
+    def __init__(self, mb: ModelBatch, theta0, spans, mode="fp32c", lr=0.05, fused=True):
         torch = _torch()
         self.mb = mb
         self.mode = mode
+        self.fused = fused
         th = np.asarray(theta0, np.float64).reshape(-1, 6)
         spans = np.maximum(np.asarray(spans, np.float64), 1e-9)
         self.base = torch.as_tensor(th[:, :4], dtype=torch.float64, device=mb.dev)
@@ -251,6 +313,8 @@ class Refiner:
         ], dim=1)
 
     def loss(self):
+        if self.fused:
+            return chi2_fused(self.theta(), self.mb, self.mode)
         return chi2_per_model(transit_flux(self.theta(), self.mb, self.mode), self.mb)
 
     def step(self):

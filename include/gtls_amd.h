@@ -27,7 +27,7 @@
  *   gtls_spectra_batched
  *       spectra() of src/gputls/stats.py:95-122 (SR, power_raw, detrended power, SDE)
  *       with the running median of src/gputls/helpers.py:91-106, per light curve.
- *   gtls_transit_fwd / gtls_transit_bwd
+ *   gtls_transit_fwd / gtls_transit_bwd / gtls_transit_chi2
  *       no reference counterpart: the model of the gradient refinement of grid peaks.
  *   gtls_set_timing / gtls_kernel_ms / gtls_set_tuning
  *       no reference counterpart: per-kernel timing and tuning of the search chain.
@@ -183,6 +183,10 @@ int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms);
  * sizes the launch; n_models <= 65535.
  * gtls_transit_bwd writes grad_theta[6 m + k] = sum_i grad_flux[out_off[m] + i] *
  * d flux_i / d theta_k, accumulated in float64.
+ * gtls_transit_chi2 fuses the model with the fit's loss and needs no flux slots: with the
+ * curves' fluxes y and weights w (float64, per point like t) it writes
+ * chi2[m] = sum_i w_i (y_i - f_i)^2 and, unless grad_theta is NULL,
+ * grad_theta[6 m + k] = d chi2[m] / d theta_k, in float64.
  */
 int gtls_transit_fwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
                      const int32_t* model_curve, const int64_t* pt_off, const double* t,
@@ -190,6 +194,9 @@ int gtls_transit_fwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double
 int gtls_transit_bwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
                      const int32_t* model_curve, const int64_t* pt_off, const double* t,
                      const int64_t* out_off, const void* grad_flux, double* grad_theta);
+int gtls_transit_chi2(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
+                      const int32_t* model_curve, const int64_t* pt_off, const double* t,
+                      const double* y, const double* w, double* chi2, double* grad_theta);
 
 #ifdef __cplusplus
 }

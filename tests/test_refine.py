@@ -156,6 +156,36 @@ def test_transit_backward_matches_autograd_and_central_differences(gpu):
 
 
 @pytest.mark.gpu
+def test_fused_chi2_matches_the_flux_path(gpu):
+    mb, th, curves, mc, _ = model_batch(seed=5)
+    theta = torch.tensor(th, device="cuda")
+    # float64 reference: the torch model, its loss and autograd gradient
+    thr = theta.clone().requires_grad_(True)
+    f = torch_flux(thr, curves, mc)
+    per_model = R.chi2_per_model(f, mb)
+    per_model.sum().backward()
+    want_c, want_g = per_model.detach(), thr.grad
+    scale = want_g.abs().amax(dim=1, keepdim=True)
+    c64, g64 = R.transit_chi2(mb, theta, "fp64")
+    assert float(((c64 - want_c).abs() / want_c).max()) < 1e-12
+    assert float(((g64 - want_g).abs() / scale).max()) < 1e-9
+    c64n, none = R.transit_chi2(mb, theta, "fp64", grad=False)
+    assert none is None and float(((c64n - c64).abs() / c64).max()) < 1e-13
+    # float32 model: the same flux values as the forward kernel's
+    c32, g32 = R.transit_chi2(mb, theta, "fp32c")
+    flux_c32 = R.chi2_per_model(R.transit_fwd(mb, theta, "fp32c"), mb)
+    assert float(((c32 - flux_c32).abs() / flux_c32).max()) < 1e-7
+    assert float(((c32 - want_c).abs() / want_c).max()) < 1e-4
+    assert float(((g32 - want_g).abs() / scale).max()) < 2e-3
+    c32n, _ = R.transit_chi2(mb, theta, "fp32c", grad=False)
+    assert float(((c32n - c32).abs() / c32).max()) < 1e-7
+    # through autograd: the same gradient as the flux kernels' chain
+    ta = theta.clone().requires_grad_(True)
+    R.chi2_fused(ta, mb, "fp64").sum().backward()
+    assert float(((ta.grad - want_g).abs() / scale).max()) < 1e-9
+
+
+@pytest.mark.gpu
 def test_autograd_function_chains_through_torch(gpu):
     mb, th, curves, mc, _ = model_batch(seed=2)
     theta = torch.tensor(th, device="cuda", requires_grad=True)
@@ -169,10 +199,11 @@ def test_autograd_function_chains_through_torch(gpu):
 
 
 @pytest.mark.gpu
-def test_refinement_recovers_injected_transits(gpu):
+@pytest.mark.parametrize("fused", [True, False])
+def test_refinement_recovers_injected_transits(gpu, fused):
     mb, th, curves, mc, truths = model_batch(seed=4, n=6000)
     spans = np.array([np.ptp(curves[c][0]) for c in mc])
-    ref = R.Refiner(mb, th, spans, mode="fp32c")
+    ref = R.Refiner(mb, th, spans, mode="fp32c", fused=fused)
     first, last = ref.run(200)
     assert torch.all(last < first)
     got = ref.theta().detach().cpu().numpy()
