@@ -74,6 +74,9 @@ def test_fold_sort_bit_exact(gpu):
         (1000 + np.arange(20000) / 720.0, np.array([0.5, 1.0, 1.5, 2.0, 3.71, 7.0])),  # ties
         (rng.uniform(-40, 40, 3000), rng.uniform(0.4, 9, 20)),                        # t < 0
         (1000 + np.arange(300) / 48.0, np.array([0.5, 0.75, 1.0])),                   # small n
+        # periods above the span: part of the phase circle is empty, so a window of the
+        # in-LDS sort spans thousands of empty buckets
+        (np.sort(rng.uniform(1000, 1010, 40000)), np.array([25.0, 31.0, 3.3])),
         # periods that are multiples of the cadence: ~139 and ~69 equal phases per bucket
         (1000 + np.arange(50000) / 720.0, np.array([0.5, 0.25, 0.5 + 1e-9])),
         # 8 distinct times repeated 600 times: 600-key buckets
