@@ -328,8 +328,8 @@ int gtls_create(int device, void* stream, gtls_ctx** out) {
     ctx->stream = (hipStream_t)stream;
     ctx->n_cu = prop.multiProcessorCount;
     // Each period chunk ends with the tails of its kernels, so chunks are made large: up
-    // to 32 GiB of per-period buffers, at most an eighth of the device memory.
-    ctx->chunk_bytes = std::min<int64_t>(32LL << 30, (int64_t)(prop.totalGlobalMem / 8));
+    // to 64 GiB of per-period buffers, at most a quarter of the device memory.
+    ctx->chunk_bytes = std::min<int64_t>(64LL << 30, (int64_t)(prop.totalGlobalMem / 4));
     if (const char* e = std::getenv("GTLS_MFMA")) ctx->use_mfma = std::atoi(e) != 0;
     if (const char* e = std::getenv("GTLS_TOEP_Q"))
         ctx->toep_q = std::max(0, std::min(kToepQ, std::atoi(e)));

@@ -16,7 +16,7 @@ from . import constants as C
 from .search import _ctx, _device, kernel_ms, set_kernel_timing
 
 TOEP_Q = (12, 16, 24)
-CHUNK_MB = (8 << 10, 32 << 10)
+CHUNK_MB = (8 << 10, 64 << 10)
 
 
 def set_tuning(device=0, toep_q=24, chunk_mb=0):

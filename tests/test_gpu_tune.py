@@ -29,4 +29,4 @@ def test_autotune_picks_a_candidate_and_keeps_results(gpu):
             rel = ((c - c0).abs() / c0.abs())[ok]
             assert float(rel.max()) < 2e-6, q
     finally:
-        tune.set_tuning(0, 24, 32 << 10)
+        tune.set_tuning(0, 24, 64 << 10)
