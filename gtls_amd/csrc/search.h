@@ -207,9 +207,11 @@ __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw
         }
     }
     __syncthreads();
+    // the windows kernel reads these only after the whole chunk is prepared: stored past
+    // the caches, so that they do not evict the curves' times and sort scratch
     for (int j = tid; j < cnt; j += kBlock) {
-        ub[o0 + j] = su[j];
-        D[o0 + j] = sD[j];
+        __builtin_nontemporal_store(su[j], &ub[o0 + j]);
+        __builtin_nontemporal_store(sD[j], &D[o0 + j]);
     }
     carry = total;
 }
