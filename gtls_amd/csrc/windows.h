@@ -231,8 +231,12 @@ __device__ __forceinline__ int ring_wrap(int r) {
 
 // K-group size of the matrix-core path: 4 M samples per group, M MFMA steps per tile. Wide
 // strides take long groups so that the MFMAs of one group cover the latency of the next
-// group's loads; short strides take short groups so that little of a group is padding.
-__host__ __device__ inline int32_t mfma_m(int32_t q) { return q >= 48 ? 8 : (q >= 16 ? 4 : 2); }
+// group's loads -- unless 32-sample groups would pad q further than 16-sample ones;
+// short strides take short groups so that little of a group is padding.
+__host__ __device__ inline int32_t mfma_m(int32_t q) {
+    if (q >= 48) return (q + 31) / 32 * 32 == (q + 15) / 16 * 16 ? 8 : 4;
+    return q >= 16 ? 4 : 2;
+}
 
 // Chunked template rows of the matrix-core path: for width w with stride q, F = d / q and
 // q_pad = q rounded up to whole K groups (4 mfma_m(q) samples), chunk c <= F sits at
