@@ -392,8 +392,7 @@ constexpr int kWinPer = (kWinCap + 255) / 256;       // staged samples per threa
 constexpr int kMaxWin = 255;                         // windows of one period (uint8 table)
 
 // Windowed stable phase sort of the n samples of t for period P (see the file comment).
-// Scratch: bk (n uint16: each sample's bucket), binned (n ints: sample indices grouped by
-// window). LDS: hist (kMaxBuckets ints), red (blockDim/64 ints), counts (4 ints), sphi /
+// Scratch: binned (n ints: sample indices grouped by window). LDS: hist (kMaxBuckets ints), red (blockDim/64 ints), counts (4 ints), sphi /
 // sidx (kWinCap doubles / ints), mids (kWinMid int64), wtab (kMaxBuckets bytes: window of
 // each bucket), wb / ws (kMaxWin + 1 ints: first bucket / first position of each window),
 // wcur (kMaxWin ints).
@@ -405,7 +404,7 @@ constexpr int kMaxWin = 255;                         // windows of one period (u
 // global form.
 template <class Sink>
 __device__ __forceinline__ bool wg_fold_sort_windows(
-    const double* __restrict__ t, int64_t n, double P, bool neg, uint16_t* __restrict__ bk,
+    const double* __restrict__ t, int64_t n, double P, bool neg,
     int32_t* __restrict__ binned, int* hist, int* red, int* counts, double* sphi,
     int32_t* sidx, int64_t* mids, uint8_t* wtab, int* wb, int* ws, int* wcur, Sink sink) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -425,9 +424,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         for (int u = 0; u < U; ++u) {
             const int64_t j = base + (int64_t)u * nt + tid;
             if (j < n) {
-                const int b = phase_bucket(fold_phase(tv[u], P), B, neg);
-                atomicAdd(&hist[b], 1);
-                bk[j] = (uint16_t)b;
+                atomicAdd(&hist[phase_bucket(fold_phase(tv[u], P), B, neg)], 1);
             }
         }
     }
@@ -488,15 +485,23 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     }
     for (int w = tid; w < K; w += nt) wcur[w] = ws[w];
     __syncthreads();
-    // one pass: every sample's index into its window's range of `binned`
+    // one pass: every sample's index into its window's range of `binned`, its bucket folded
+    // again (the times are shared by the curve's periods and stay in cache; a stored bucket
+    // would be per-period scratch traffic)
     {
         constexpr int UW = 16;
         for (int64_t base = 0; base < n; base += (int64_t)UW * nt) {
             int bv[UW];
+            double tq[UW];
 #pragma unroll
             for (int u = 0; u < UW; ++u) {
                 const int64_t j = base + (int64_t)u * nt + tid;
-                bv[u] = j < n ? (int)bk[j] : -1;
+                tq[u] = j < n ? t[j] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < UW; ++u) {
+                const int64_t j = base + (int64_t)u * nt + tid;
+                bv[u] = j < n ? phase_bucket(fold_phase(tq[u], P), B, neg) : -1;
             }
 #pragma unroll
             for (int u = 0; u < UW; ++u) {

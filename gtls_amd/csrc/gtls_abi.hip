@@ -69,7 +69,7 @@ struct SearchLayout {
     int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, tmplsq = 0, ss = 0, wconst = 0, flat = 0, counters = 0;
     int64_t tmplc1 = 0, tmplc2 = 0, tzero = 0, n_chunked = 0, n_zero = 0;
     int64_t slots = 0, slot_bytes = 0, o_key = 0, o_perm = 0, o_tmp_key = 0, o_tmp_id = 0;
-    int64_t o_lists = 0, o_bk = 0, n_slots = 0;
+    int64_t o_lists = 0, n_slots = 0;
     int64_t xbuf = 0, dbuf = 0, meta = 0, part = 0;
     int64_t xstride = 0, chunk = 0, n_chunks = 0, kmax = 0, n_win = 0;
     int64_t total = 0;
@@ -122,8 +122,6 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     s = align_up(s + 4 * max_points);
     L.o_lists = s;
     s = align_up(s + 8 * (int64_t)kMaxBuckets);
-    L.o_bk = s;
-    s = align_up(s + 2 * max_points);
     L.slot_bytes = s;
     L.n_slots = (mode == GTLS_MODE_FP64) ? ctx->prep_fp64 : ctx->prep_fp32;
     L.n_win = (mode == GTLS_MODE_FP64) ? ctx->win_fp64 : ctx->win_fp32;
@@ -257,7 +255,6 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.o_tmp_key = L.o_tmp_key;
     p.o_tmp_id = L.o_tmp_id;
     p.o_lists = L.o_lists;
-    p.o_bk = L.o_bk;
     p.xbuf = (R2*)(ws + L.xbuf);
     p.dbuf = (double*)(ws + L.dbuf);
     p.xstride = L.xstride;

@@ -74,7 +74,7 @@ struct SearchParams {
     // prep_kernel's per-workgroup sort scratch
     char* slots;
     int64_t slot_bytes;
-    int64_t o_key, o_perm, o_tmp_key, o_tmp_id, o_lists, o_bk;
+    int64_t o_key, o_perm, o_tmp_key, o_tmp_id, o_lists;
     // per chunk period i: gathered samples in xbuf[i * xstride ...] (room for 2 xstride
     // Reals: u, or the planes v = u w at [0, xstride) and w at [xstride, 2 xstride)) and
     // their float64 running sum dbuf[i * xstride ...]
@@ -240,7 +240,6 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
     double* tmp_key = (double*)(slot + p.o_tmp_key);
     int32_t* tmp_id = (int32_t*)(slot + p.o_tmp_id);
     int64_t* lists = (int64_t*)(slot + p.o_lists);
-    uint16_t* bk = (uint16_t*)(slot + p.o_bk);
 
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&p.counters[0], 1ULL);
@@ -283,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
             const R2* uw = p.uw0 + pt0;
             double carry = 0.0;
             const bool windowed = wg_fold_sort_windows(
-                p.t + pt0, n, P, p.negative_time != 0, bk, perm, hist, red_i, s_counts, s_phi,
+                p.t + pt0, n, P, p.negative_time != 0, perm, hist, red_i, s_counts, s_phi,
                 s_idx, s_mids, s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                     if constexpr (sizeof(Real) == 4) {
                         if (constw) {
@@ -519,8 +518,8 @@ __global__ void reduce_kernel(SearchParams<Real> p) {
 // (8 n), merge scratch (8 n + 4 n), bucket lists (8 kMaxBuckets), bucket of each sample
 // (2 n).
 __host__ __device__ inline int64_t fold_sort_slot_bytes(int64_t n) {
-    return ((8 * n + 255) / 256 + (12 * n + 255) / 256 + (8 * (int64_t)kMaxBuckets + 255) / 256 +
-            (2 * n + 255) / 256) * 256;
+    return ((8 * n + 255) / 256 + (12 * n + 255) / 256 + (8 * (int64_t)kMaxBuckets + 255) / 256) *
+           256;
 }
 
 __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int64_t n,
@@ -540,11 +539,10 @@ __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int6
     double* tmp_key = (double*)(slot + (8 * n + 255) / 256 * 256);
     int32_t* tmp_id = (int32_t*)(tmp_key + n);
     int64_t* lists = (int64_t*)(slot + (8 * n + 255) / 256 * 256 + (12 * n + 255) / 256 * 256);
-    uint16_t* bk = (uint16_t*)((char*)lists + (8 * (int64_t)kMaxBuckets + 255) / 256 * 256);
     for (int64_t pi = blockIdx.x; pi < n_periods; pi += gridDim.x) {
         int32_t* out = perm_out + pi * n;
         const bool windowed = wg_fold_sort_windows(
-            t, n, periods[pi], neg != 0, bk, tmp_id, hist, red_i, s_counts, s_phi, s_idx, s_mids,
+            t, n, periods[pi], neg != 0, tmp_id, hist, red_i, s_counts, s_phi, s_idx, s_mids,
             s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                 for (int j = threadIdx.x; j < wcnt; j += blockDim.x) out[wstart + j] = s_idx[j];
             });
