@@ -219,6 +219,7 @@ __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw
 // Stage 1 + 2 of each period of the chunk, one workgroup per period (persistent):
 //   1. fold + stable phase sort (fold_sort.h)                  -> perm
 //   2. gather in phase order with the wrap-around extension    -> X, running sum D
+// Two workgroups per CU (their LDS allows no more) at 256 registers: two waves per SIMD.
 template <class Real>
 __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
     using R2 = typename Pair<Real>::type;
