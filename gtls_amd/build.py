@@ -3,7 +3,9 @@
     python -m gtls_amd.build
 
 hipcc cross-compiles for gfx950 on a machine without a GPU. Packed float32 instructions
-are switched off for the device code (-packed-fp32-ops), see DESIGN.md.
+(v_pk_add/mul/fma_f32) are switched off for the device code (-packed-fp32-ops): some of
+their forms have given wrong results on these GPUs without any error, so the compiler
+must not form them from ordinary float code.
 """
 from __future__ import annotations
 
@@ -15,7 +17,6 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libgtls_amd.so")
 SOURCES = [os.path.join(PKG, "csrc", "gtls_abi.hip")]
-HEADERS = [os.path.join(PKG, "csrc", f) for f in ("search.h", "fold_sort.h", "wg_primitives.h")]
 
 
 def hipcc():
