@@ -2,10 +2,12 @@
 
 Each grid peak of a light curve (period, best width and T0 position of the search) seeds a
 model with parameters theta = (P, T0, duration, depth, q1, q2); the models are fitted to
-their curves by Adam on the weighted chi2 sum_i w_i (y_i - f_i)^2. The model flux and its
-derivative run in the HIP kernels behind gtls_transit_fwd / gtls_transit_bwd
-(csrc/transit_model.h); `TransitModel` wraps them as one torch.autograd.Function, so the
-loss and the parameter transform are ordinary torch code.
+their curves by Adam on the weighted chi2 sum_i w_i (y_i - f_i)^2. The per-model chi2 and
+its gradient come from one fused HIP kernel pass (gtls_transit_chi2, wrapped as the
+torch.autograd.Function `TransitChi2`); for losses of the caller's own, the model flux of
+every point and its backward run in gtls_transit_fwd / gtls_transit_bwd (`TransitModel`).
+All three kernels are in csrc/transit_model.h; the parameter transform is ordinary torch
+code.
 
 The reference has no such step: it reports the grid values refined on the host
 (src/gputls/stats.py:233-488). `transit_model_torch` restates the model in torch
