@@ -187,6 +187,8 @@ def transit_chi2(mb: ModelBatch, theta, mode="fp32c", grad=True):
     theta = theta.detach().to(torch.float64).contiguous()
     if theta.shape != (mb.n_models, 6) or theta.device != mb.dev:
         raise ValueError("theta must be a float64 [n_models, 6] tensor on the batch's device")
+    if mb.y_pt is None or mb.w_pt is None:
+        raise ValueError("the batch has no per-point flux and weights (ModelBatch.build sets them)")
     chi2 = torch.empty(mb.n_models, dtype=torch.float64, device=mb.dev)
     g = torch.empty((mb.n_models, 6), dtype=torch.float64, device=mb.dev) if grad else None
     ctx = _ctx(mb.dev)
