@@ -54,9 +54,11 @@ __device__ __forceinline__ bool key_greater(double ka, int32_t ia, double kb, in
     return ka > kb || (ka == kb && ia > ib);
 }
 
-// Compare-exchange of two (phase, index) keys held in registers.
+// Compare-exchange of two (phase, index) keys held in registers; with PhaseOnly, on the
+// phase alone (equal phases may then end in either order).
+template <bool PhaseOnly = false>
 __device__ __forceinline__ void cmp_swap(double& ka, int32_t& ia, double& kb, int32_t& ib) {
-    const bool sw = key_greater(ka, ia, kb, ib);
+    const bool sw = PhaseOnly ? ka > kb : key_greater(ka, ia, kb, ib);
     const double k0 = sw ? kb : ka, k1 = sw ? ka : kb;
     const int32_t i0 = sw ? ib : ia, i1 = sw ? ia : ib;
     ka = k0; kb = k1; ia = i0; ib = i1;
@@ -64,8 +66,9 @@ __device__ __forceinline__ void cmp_swap(double& ka, int32_t& ia, double& kb, in
 
 // Batcher's odd-even merge network for 8 keys (19 compare-exchanges); unused slots hold
 // +inf keys, which sort last.
+template <bool PhaseOnly = false>
 __device__ __forceinline__ void sort8(double* k, int32_t* id) {
-#define GTLS_CS(a, b) cmp_swap(k[a], id[a], k[b], id[b])
+#define GTLS_CS(a, b) cmp_swap<PhaseOnly>(k[a], id[a], k[b], id[b])
     GTLS_CS(0, 1); GTLS_CS(2, 3); GTLS_CS(4, 5); GTLS_CS(6, 7);
     GTLS_CS(0, 2); GTLS_CS(1, 3); GTLS_CS(4, 6); GTLS_CS(5, 7);
     GTLS_CS(1, 2); GTLS_CS(5, 6);
@@ -76,8 +79,9 @@ __device__ __forceinline__ void sort8(double* k, int32_t* id) {
 }
 
 // Batcher's odd-even merge network for 16 keys (63 compare-exchanges).
+template <bool PhaseOnly = false>
 __device__ __forceinline__ void sort16(double* k, int32_t* id) {
-#define GTLS_CS(a, b) cmp_swap(k[a], id[a], k[b], id[b])
+#define GTLS_CS(a, b) cmp_swap<PhaseOnly>(k[a], id[a], k[b], id[b])
     GTLS_CS(0, 1); GTLS_CS(2, 3); GTLS_CS(0, 2); GTLS_CS(1, 3); GTLS_CS(1, 2);
     GTLS_CS(4, 5); GTLS_CS(6, 7); GTLS_CS(4, 6); GTLS_CS(5, 7); GTLS_CS(5, 6);
     GTLS_CS(0, 4); GTLS_CS(2, 6); GTLS_CS(2, 4); GTLS_CS(1, 5); GTLS_CS(3, 7);
@@ -106,8 +110,17 @@ __device__ __forceinline__ void sort_bucket_regs(double* __restrict__ key, int32
         k[e] = in ? key[lo + (in ? e : 0)] : __builtin_inf();
         id[e] = in ? perm[lo + (in ? e : 0)] : 0x7fffffff;
     }
-    if constexpr (M == 8) sort8(k, id);
-    else sort16(k, id);
+    // on the phases alone, then again on the full key only when two of the bucket's
+    // phases are equal (their order is then decided by the index)
+    if constexpr (M == 8) sort8<true>(k, id);
+    else sort16<true>(k, id);
+    bool tie = false;
+#pragma unroll
+    for (int e = 0; e + 1 < M; ++e) tie |= (e + 1 < cnt) && k[e] == k[e + 1];
+    if (tie) {
+        if constexpr (M == 8) sort8(k, id);
+        else sort16(k, id);
+    }
 #pragma unroll
     for (int e = 0; e < M; ++e) {
         if (e < cnt) {
