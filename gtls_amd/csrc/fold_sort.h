@@ -55,11 +55,12 @@ __device__ __forceinline__ bool key_greater(double ka, int32_t ia, double kb, in
 }
 
 // Compare-exchange of two (phase, index) keys held in registers; with PhaseOnly, on the
-// phase alone (equal phases may then end in either order).
+// phase alone (equal phases may then end in either order), the phases placed by min/max.
 template <bool PhaseOnly = false>
 __device__ __forceinline__ void cmp_swap(double& ka, int32_t& ia, double& kb, int32_t& ib) {
     const bool sw = PhaseOnly ? ka > kb : key_greater(ka, ia, kb, ib);
-    const double k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    const double k0 = PhaseOnly ? fmin(ka, kb) : (sw ? kb : ka);
+    const double k1 = PhaseOnly ? fmax(ka, kb) : (sw ? ka : kb);
     const int32_t i0 = sw ? ib : ia, i1 = sw ? ia : ib;
     ka = k0; kb = k1; ia = i0; ib = i1;
 }
