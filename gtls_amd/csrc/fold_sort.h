@@ -557,12 +557,13 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         for (int b = bfirst + tid; b < bnext; b += nt) {
             const int blo = (b > bfirst ? hist[b - 1] : wstart) - wstart;
             const int cnt = hist[b] - wstart - blo;
-            if (cnt <= 1) continue;
-            if (cnt <= 8) {
-                sort_bucket_regs<8>(sphi, sidx, blo, cnt);
-            } else if (cnt <= 16) {
-                sort_bucket_regs<16>(sphi, sidx, blo, cnt);
-            } else {
+            // one network per pass of the wavefront: the 16-key one for all of its lanes
+            // when any of them holds 9-16 keys (lanes would otherwise run both networks)
+            const bool wide = __ballot(cnt > 8 && cnt <= 16) != 0;
+            if (cnt > 1 && cnt <= 16) {
+                if (wide) sort_bucket_regs<16>(sphi, sidx, blo, cnt);
+                else sort_bucket_regs<8>(sphi, sidx, blo, cnt);
+            } else if (cnt > 16) {
                 const int e = atomicAdd(&counts[1], 1);
                 mids[e] = ((int64_t)blo << 32) | (int64_t)cnt;
             }
