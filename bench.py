@@ -183,8 +183,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build searches a batch on a single device "
-              "and reports no multi-GPU figure; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the search runs each batch on a single device "
+              "and there is no multi-GPU figure to report; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.curves % args.batch:
         print("--curves must be a multiple of --batch", file=sys.stderr)
