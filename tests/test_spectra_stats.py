@@ -125,3 +125,43 @@ def test_candidate_indices_top_then_long_periods():
     rest = [i for i in order if i not in set(order[:100]) and periods[i] > 1][:100]
     assert nxt == rest
     assert nearest_indices([0.5, 20.0, 10.0], periods) == [0, 999, int(np.argmin(abs(periods - 10)))]
+
+
+def test_refine_period_box_transit_closed_forms():
+    """refine_period end to end on a box transit of known period, T0, duration and depth,
+    from the oracle's best (width, T0) of that period: transit times P apart across the
+    span, T0 and duration to a few cadences, and depth = delta * overshoot * (box samples in
+    the window / window samples)."""
+    from gtls_amd import search as S
+    from gtls_amd.validate import validate_args
+    from oracle import oracle
+    rng = np.random.default_rng(3)
+    dt = 2 / 1440
+    t = 1000 + np.arange(6000) * dt
+    P, T0, D, delta = 1.7, 1000.5, 0.1, 0.01
+    ph = ((t - T0 + P / 2) % P) - P / 2
+    inbox = np.abs(ph) < D / 2
+    y = 1 - delta * inbox + rng.normal(0, 1e-5, t.size)
+    dy = np.full(t.size, 1e-3)
+    plan = S.make_plan(t, y, dy, validate_args({}), periods=[P])
+    tb = plan.table
+    _, _, k, s = oracle.period_table(plan.t, plan.y, plan.dy, P, tb.widths, tb.S, tb.offsets,
+                                     tb.overshoot, plan.k_lo[0], plan.k_hi[0], skip_point=8)
+    r = ST.refine_period(plan, P, k, s)
+    tt = np.asarray(r["transitTimes"])
+    assert np.allclose(np.diff(tt), P, rtol=0, atol=1e-9)
+    assert t.min() <= tt[0] < t.min() + P and tt[-1] < t.max()
+    assert len(tt) == int(np.floor((t.max() - tt[0]) / P)) + 1
+    dT0 = ((r["T0"] - T0 + P / 2) % P) - P / 2
+    assert abs(dT0) <= 4 * dt
+    assert abs(r["duration"] - D) <= 4 * dt
+    width = int(tb.widths[k])
+    assert r["durationPoints"] == width and width >= inbox.sum()
+    # the window's mean holds n_in box samples of depth delta: depth / (delta overshoot)
+    # times the width is an integer, at most the folded box and short of it only by the
+    # samples of a few-cadence T0 offset
+    n_in = r["depth"] / (delta * tb.overshoot[k]) * width
+    assert abs(n_in - round(n_in)) < 0.1
+    assert inbox.sum() - 4 * len(tt) <= round(n_in) <= inbox.sum()
+    for key in ("snr", "snrPink", "snrFit", "snrFitPink"):
+        assert np.isfinite(r[key]) and r[key] > 0
