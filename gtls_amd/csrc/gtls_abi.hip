@@ -629,8 +629,9 @@ int gtls_transit_chi2(gtls_ctx* ctx, int32_t mode, int64_t n_models, const doubl
     if (!theta || !model_curve || !pt_off || !t || !y || !w || !chi2)
         return fail(ctx, GTLS_E_SHAPE, "null transit model argument");
     DeviceGuard guard(ctx->device);
-    const dim3 grid((unsigned)n_models), block(256);
+    const dim3 grid((unsigned)n_models);
     if (mode == GTLS_MODE_FP64) {
+        const dim3 block(chi2_block<double>());
         if (grad_theta)
             hipLaunchKernelGGL((transit_chi2_kernel<double, true>), grid, block, 0, ctx->stream,
                                theta, model_curve, pt_off, t, y, w, chi2, grad_theta);
@@ -638,6 +639,7 @@ int gtls_transit_chi2(gtls_ctx* ctx, int32_t mode, int64_t n_models, const doubl
             hipLaunchKernelGGL((transit_chi2_kernel<double, false>), grid, block, 0, ctx->stream,
                                theta, model_curve, pt_off, t, y, w, chi2, grad_theta);
     } else {
+        const dim3 block(chi2_block<float>());
         if (grad_theta)
             hipLaunchKernelGGL((transit_chi2_kernel<float, true>), grid, block, 0, ctx->stream,
                                theta, model_curve, pt_off, t, y, w, chi2, grad_theta);

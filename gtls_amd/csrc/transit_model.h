@@ -168,10 +168,14 @@ __global__ __launch_bounds__(256) void transit_bwd_kernel(const double* __restri
 // points of curve c = model_curve[m] (times t, fluxes y, weights w, all float64),
 //     chi2[m] = sum_i w_i (y_i - f_i)^2,
 //     grad_theta[6 m + k] = sum_i -2 w_i (y_i - f_i) df_i/dtheta_k   (when Grad).
-// One workgroup per model; the residuals and sums are float64 (f in Real). Each thread has
-// kBwdUnroll points' loads in flight.
+// One wide workgroup per model (chi2_block threads: the longest curves then leave no tail
+// of latency-bound workgroups; 512 for the float64 model, whose derivatives need more
+// registers); the residuals and sums are float64 (f in Real). Each thread has kBwdUnroll
+// points' loads in flight.
+template <class Real>
+constexpr int chi2_block() { return sizeof(Real) == 4 ? 1024 : 512; }
 template <class Real, bool Grad>
-__global__ __launch_bounds__(256) void transit_chi2_kernel(const double* __restrict__ theta,
+__global__ __launch_bounds__(chi2_block<Real>()) void transit_chi2_kernel(const double* __restrict__ theta,
                                                            const int32_t* __restrict__ model_curve,
                                                            const int64_t* __restrict__ pt_off,
                                                            const double* __restrict__ t,
@@ -180,7 +184,7 @@ __global__ __launch_bounds__(256) void transit_chi2_kernel(const double* __restr
                                                            double* __restrict__ chi2,
                                                            double* __restrict__ grad_theta) {
     constexpr int kSums = Grad ? 7 : 1;
-    __shared__ double red[kSums][256 / kWave];
+    __shared__ double red[kSums][chi2_block<Real>() / kWave];
     const int64_t m = blockIdx.x;
     const int c = model_curve[m];
     const int64_t a = pt_off[c], n = pt_off[c + 1] - a;
