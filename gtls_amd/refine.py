@@ -302,7 +302,7 @@ class Refiner:
         z[:, 4] = _logit(th[:, 4])
         z[:, 5] = _logit(th[:, 5])
         self.z = torch.tensor(z, dtype=torch.float64, device=mb.dev, requires_grad=True)
-        self.opt = torch.optim.Adam([self.z], lr=lr)
+        self.opt = torch.optim.Adam([self.z], lr=lr, fused=True)
 
     def theta(self):
         torch = _torch()
