@@ -228,24 +228,47 @@ class DeviceBatch:
 
     def capture(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
         """A HIP graph (torch.cuda.CUDAGraph) of search(): replay() re-runs every size
-        group's search call into chi2 / best_k / best_s with no host work. The search runs
-        once on the graph's stream first, so that its library context and workspace exist
+        group's search call into chi2 / best_k / best_s with no host work. The graph gets a
+        workspace of its own, allocated before the capture and kept alive with the graph
+        (attribute `workspace` of the returned graph, and this batch's `graph_workspaces`),
+        so no later search or capture can free or share the memory its kernels write. The
+        search runs once on the graph's stream first, so that the library context exists
         before the capture; every kernel of the call is captured (no memset nodes)."""
         import torch
+        ws = torch.empty(self.workspace_bytes(mode, skip_point, depth_min), dtype=torch.uint8,
+                         device=self.dev)
         s = torch.cuda.Stream(self.dev)
         s.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(s):
-            self.search(mode, skip_point, depth_min)
+            self.search(mode, skip_point, depth_min, ws=ws)
         torch.cuda.current_stream(self.dev).wait_stream(s)
         torch.cuda.synchronize(self.dev)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=s):
-            self.search(mode, skip_point, depth_min)
+            self.search(mode, skip_point, depth_min, ws=ws)
+        g.workspace = ws
+        self.graph_workspaces = getattr(self, "graph_workspaces", []) + [ws]
         return g
 
-    def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
+    def workspace_bytes(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
+        """Workspace bytes of search(): the largest need over the size groups."""
+        m = MODES[mode]
+        ctx = _ctx(self.dev)
+        lib = _ffi.load()
+        need = 0
+        for g in self.groups:
+            b = self.batch_struct(skip_point, depth_min, g)
+            nb = lib.gtls_search_workspace_bytes(ctx.handle, m, ct.byref(b))
+            if nb < 0:
+                _ffi.check(ctx.handle, int(nb))
+            need = max(need, int(nb))
+        return need
+
+    def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, ws=None):
         """Enqueue the batched search on torch's current stream (no synchronisation): one
-        gtls_search_batched call per size group, each writing its slice of the outputs."""
+        gtls_search_batched call per size group, each writing its slice of the outputs.
+        `ws` (a uint8 device tensor of at least workspace_bytes()) is the workspace; by
+        default the one shared by searches on the current stream."""
         m = MODES[mode]
         ctx = _ctx(self.dev)
         lib = _ffi.load()
@@ -256,7 +279,10 @@ class DeviceBatch:
             if nb < 0:
                 _ffi.check(ctx.handle, int(nb))
             need = max(need, int(nb))
-        ws = workspace(self.dev, need)
+        if ws is None:
+            ws = workspace(self.dev, need)
+        elif ws.numel() < need:
+            raise ValueError("workspace of %d bytes, the search needs %d" % (ws.numel(), need))
         for g, b in zip(self.groups, structs):
             if b.total_periods == 0:
                 continue

@@ -463,7 +463,13 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         if (mx > kWaveSortMax) atomicOr(&counts[0], 1);
     }
     __syncthreads();
-    if (counts[0] != 0) return false;
+    // Every thread reads the flag before any thread leaves: the caller's global form resets
+    // counts[] before its first barrier, so a late read after a return could see 0.
+    {
+        const bool too_big = counts[0] != 0;
+        __syncthreads();
+        if (too_big) return false;
+    }
     // the windows: buckets wb[w] .. wb[w + 1] - 1, wb[w + 1] the first bucket starting at or
     // past ws[w] + kWinTarget, so that a window holds fewer than kWinCap samples
     if (tid == 0) {
@@ -487,8 +493,12 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         counts[3] = bfirst < B ? 1 : 0;   // more windows needed than the table holds
     }
     __syncthreads();
-    if (counts[3] != 0) return false;
     const int K = counts[2];
+    {
+        const bool too_many = counts[3] != 0;
+        __syncthreads();   // as above: all reads of counts[] done before a return
+        if (too_many) return false;
+    }
     for (int b = tid; b < B; b += nt) {
         int lo = 0, hi = K;   // last w with wb[w] <= b
         while (hi - lo > 1) {

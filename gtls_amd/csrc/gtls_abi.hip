@@ -30,6 +30,7 @@ struct gtls_ctx {
     int timing = 0;
     std::vector<hipEvent_t> pool;       // events owned by the context
     size_t used = 0;                    // events recorded since the last gtls_kernel_ms
+    bool overflow = false;              // more records than kMaxTimingEvents were asked for
     ~gtls_ctx() {
         for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     }
@@ -40,6 +41,7 @@ namespace {
 using namespace gtls;
 
 constexpr int64_t kAlign = 256;
+constexpr size_t kMaxTimingEvents = 1 << 16;   // records kept between gtls_kernel_ms calls
 
 int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -67,6 +69,7 @@ int hip_fail(gtls_ctx* ctx, hipError_t e, const char* where) {
 
 struct SearchLayout {
     int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, tmplsq = 0, ss = 0, wconst = 0, flat = 0, counters = 0;
+    int64_t tmpl_tail = 0;
     int64_t tmplc1 = 0, tmplc2 = 0, tzero = 0, n_chunked = 0, n_zero = 0;
     int64_t slots = 0, slot_bytes = 0, o_key = 0, o_perm = 0, o_tmp_key = 0, o_tmp_id = 0;
     int64_t o_lists = 0, n_slots = 0;
@@ -84,9 +87,10 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     int64_t off = 0;
     L.uw0 = off;
     off = align_up(off + 2 * real * b->total_points);
-    // rows with kTmplPad zeros on both sides, then a zeroed tail for the over-reads of the
-    // matrix-core path (up to one block, i.e. max_width samples, past a row)
-    const int64_t tmpl_padded = b->total_tmpl + 2 * kTmplPad * b->total_widths + kTmplTail;
+    // rows with kTmplPad zeros on both sides, then a zeroed tail of at least one widest
+    // row past the last row (over-reads of the window loops)
+    L.tmpl_tail = std::max<int64_t>(kTmplTail, align_up(max_width + 1024));
+    const int64_t tmpl_padded = b->total_tmpl + 2 * kTmplPad * b->total_widths + L.tmpl_tail;
     L.tmpl2 = off;
     off = align_up(off + 2 * real * tmpl_padded);
     L.tmpl1 = off;
@@ -167,10 +171,6 @@ bool batch_ok(const gtls_batch* b, std::string& why) {
         why = "a light curve or width exceeds 2^30 samples";
         return false;
     }
-    if (b->max_width > kTmplTail - 64) {
-        why = "a template width exceeds the engine's limit of 32704 samples";
-        return false;
-    }
     if (b->skip_point <= 0) { why = "skip_point must be positive"; return false; }
     if (b->max_period_widths <= 0 || b->max_period_widths > (1 << 20)) {
         why = "max_period_widths out of range";
@@ -203,8 +203,8 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     hipLaunchKernelGGL(prep_points_kernel<Real>, dim3((unsigned)((b->total_points + tpb - 1) / tpb)),
                        dim3(tpb), 0, st, b->y, b->dy, b->total_points, uw0);
     hipLaunchKernelGGL(prep_template_kernel<Real>, dim3((unsigned)b->total_widths + 8), dim3(tpb),
-                       0, st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->total_tmpl, tmpl2,
-                       tmpl1, tmplsq);
+                       0, st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->total_tmpl,
+                       L.tmpl_tail, tmpl2, tmpl1, tmplsq);
     if (L.n_chunked > 0)
         hipLaunchKernelGGL(chunk_template_kernel, dim3((unsigned)b->total_widths + 4), dim3(tpb), 0,
                            st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->skip_point,
@@ -264,8 +264,17 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.best_k = best_k;
     p.best_s = best_s;
     // with timing on, an event before and after each of the chunk's three kernels
+    // Events are not recorded while the stream is being captured into a graph (they would
+    // become graph nodes that are never timed), nor past kMaxTimingEvents records.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (ctx->timing) {
+        const hipError_t ce = hipStreamIsCapturing(st, &cap);
+        if (ce != hipSuccess) return hip_fail(ctx, ce, "stream capture status");
+    }
+    const bool timed = ctx->timing && cap == hipStreamCaptureStatusNone;
+    if (timed && ctx->used + 4 * (size_t)L.n_chunks > kMaxTimingEvents) ctx->overflow = true;
     auto mark = [&]() -> hipError_t {
-        if (!ctx->timing) return hipSuccess;
+        if (!timed || ctx->overflow) return hipSuccess;
         if (ctx->used == ctx->pool.size()) {
             hipEvent_t e;
             const hipError_t r = hipEventCreate(&e);
@@ -366,6 +375,7 @@ int gtls_set_timing(gtls_ctx* ctx, int32_t on) {
     if (!ctx) return GTLS_E_SHAPE;
     ctx->timing = on != 0;
     ctx->used = 0;
+    ctx->overflow = false;
     return GTLS_OK;
 }
 
@@ -373,6 +383,12 @@ int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms) {
     if (!ctx) return GTLS_E_SHAPE;
     if (!ms) return fail(ctx, GTLS_E_SHAPE, "NULL pointer");
     ms[0] = ms[1] = ms[2] = 0.0;
+    if (ctx->overflow) {
+        ctx->used = 0;
+        ctx->overflow = false;
+        return fail(ctx, GTLS_E_SHAPE, "more than 16384 period chunks timed since the last "
+                                       "gtls_kernel_ms call; records discarded");
+    }
     const size_t n = ctx->used / 4 * 4;
     DeviceGuard guard(ctx->device);
     if (n > 0) {

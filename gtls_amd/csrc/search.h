@@ -23,7 +23,7 @@
 namespace gtls {
 
 constexpr int kBlock = 256;
-constexpr int64_t kTmplTail = 32768;  // zeroed samples after the last template row
+constexpr int64_t kTmplTail = 32768;  // least zeroed samples after the last template row
 constexpr int kScanPerThread = 11;  // gathered elements per thread per prefix-scan chunk
 static_assert(kBlock * kScanPerThread >= kWinCap, "one prefix-scan chunk per sort window");
 
@@ -572,14 +572,14 @@ __global__ void prep_points_kernel(const double* y, const double* dy, int64_t n,
 // kTmplPad zeros: row w starts at tmpl_off[w] + 2 kTmplPad w + kTmplPad. One block per row.
 template <class Real>
 __global__ void prep_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
-                                     int64_t nw, int64_t total_tmpl,
+                                     int64_t nw, int64_t total_tmpl, int64_t tail,
                                      typename Pair<Real>::type* out, Real* out1, Real* outsq) {
     const int64_t w = blockIdx.x;
     if (w >= nw) {
         // the zeroed tail after the last row (over-reads of the matrix-core path)
         const int64_t end = total_tmpl + 2 * kTmplPad * nw;
         for (int64_t i = end + (int64_t)(w - nw) * blockDim.x + threadIdx.x;
-             i < end + kTmplTail; i += (int64_t)(gridDim.x - nw) * blockDim.x) {
+             i < end + tail; i += (int64_t)(gridDim.x - nw) * blockDim.x) {
             typename Pair<Real>::type o;
             o.x = (Real)0;
             o.y = (Real)0;

@@ -31,3 +31,32 @@ def test_graph_replays_match_eager_search(gpu):
             assert torch.equal(got[1], want[1]) and torch.equal(got[2], want[2]), (mode, r)
             other.search(mode, 8)            # eager work between replays
             torch.cuda.synchronize()
+
+
+def test_two_graphs_keep_their_own_workspaces(gpu):
+    """Graphs captured from two batches, with an eager search that needs a larger shared
+    workspace run between the replays: each graph writes only into the workspace it owns,
+    so both keep reproducing their eager results."""
+    small = B.DeviceBatch(B.BatchPlan(B.build_plans(
+        synthetic.batch(4, seed=21, n_min=2000, n_max=3000), {})), 0)
+    large = B.DeviceBatch(B.BatchPlan(B.build_plans(
+        synthetic.batch(6, seed=22, n_min=5000, n_max=9000), {})), 0)
+    bigger = B.DeviceBatch(B.BatchPlan(B.build_plans(
+        synthetic.batch(3, seed=23, n_min=15000, n_max=20000), {})), 0)
+    want = {}
+    for name, db in (("small", small), ("large", large)):
+        want[name] = [x.clone() for x in db.search("fp32c", 8)]
+    torch.cuda.synchronize()
+    graphs = {"small": small.capture("fp32c", 8), "large": large.capture("fp32c", 8)}
+    assert graphs["small"].workspace.data_ptr() != graphs["large"].workspace.data_ptr()
+    for r in range(2):
+        for name, db in (("small", small), ("large", large)):
+            db.chi2.fill_(float("nan"))
+            torch.cuda.synchronize()
+            graphs[name].replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(db.chi2.cpu().numpy(), want[name][0].cpu().numpy(),
+                                  equal_nan=True), (name, r)
+            assert torch.equal(db.best_k, want[name][1]) and torch.equal(db.best_s, want[name][2])
+            bigger.search("fp32c", 8)    # grows (reallocates) the shared workspace
+            torch.cuda.synchronize()
