@@ -513,7 +513,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     // again (the times are shared by the curve's periods and stay in cache; a stored bucket
     // would be per-period scratch traffic)
     {
-        constexpr int UW = 24;   // samples in flight per thread (32 spills into AGPRs)
+        constexpr int UW = 20;   // samples in flight per thread (24 or more spill into AGPRs)
         for (int64_t base = 0; base < n; base += (int64_t)UW * nt) {
             int bv[UW];
             double tq[UW];

@@ -70,7 +70,7 @@ int hip_fail(gtls_ctx* ctx, hipError_t e, const char* where) {
 struct SearchLayout {
     int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, tmplsq = 0, ss = 0, wconst = 0, flat = 0, counters = 0;
     int64_t tmpl_tail = 0;
-    int64_t tmplc1 = 0, tmplc2 = 0, tzero = 0, n_chunked = 0, n_zero = 0;
+    int64_t tmplh = 0, tmplc = 0, xsc = 0, n_chunked = 0, n_zero = 0, tplen = 0, cplen = 0;
     int64_t slots = 0, slot_bytes = 0, o_key = 0, o_perm = 0, o_tmp_key = 0, o_tmp_id = 0;
     int64_t o_lists = 0, n_slots = 0;
     int64_t xbuf = 0, dbuf = 0, meta = 0, part = 0;
@@ -97,17 +97,20 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     off = align_up(off + real * tmpl_padded);
     L.tmplsq = off;
     off = align_up(off + real * tmpl_padded);
-    // float32: chunked rows of the matrix-core path and a block of zeros
+    // float32: float16 pair planes of the template rows (Toeplitz tiles, five planes) and of
+    // the chunked rows with their zero blocks (diagonal-sum tiles, four planes)
     if (mode == GTLS_MODE_FP32C) {
-        L.n_chunked = 2 * b->total_tmpl + 1024 * b->total_widths + 1024;
+        L.tplen = tmpl_padded;
+        L.n_chunked = 4 * b->total_tmpl + 1024 * b->total_widths + 1024;
         L.n_zero = max_width + 128;
+        L.cplen = L.n_chunked + L.n_zero;
     }
-    L.tmplc1 = off;
-    off = align_up(off + 4 * L.n_chunked);
-    L.tmplc2 = off;
-    off = align_up(off + 4 * L.n_chunked);
-    L.tzero = off;
-    off = align_up(off + 4 * L.n_zero);
+    L.tmplh = off;
+    off = align_up(off + 20 * L.tplen);
+    L.tmplc = off;
+    off = align_up(off + 16 * L.cplen);
+    L.xsc = off;
+    off = align_up(off + 8 * (int64_t)b->n_curves);
     L.ss = off;
     off = align_up(off + 8 * (b->total_widths > 0 ? b->total_widths : 1));
     L.wconst = off;
@@ -179,6 +182,17 @@ bool batch_ok(const gtls_batch* b, std::string& why) {
     return true;
 }
 
+// The float32 tiles address planes with 32-bit byte offsets.
+bool layout_ok(const SearchLayout& L, int mode, std::string& why) {
+    if (mode != GTLS_MODE_FP32C) return true;
+    if (4 * L.tplen >= (1LL << 30) || 4 * L.cplen >= (1LL << 30) || 8 * L.xstride >= (1LL << 32)) {
+        why = "batch too large for the float32 search (template planes over 2^30 bytes or a "
+              "light curve over 2^29 samples): split it";
+        return false;
+    }
+    return true;
+}
+
 __global__ void zero_u64s_kernel(unsigned long long* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -204,16 +218,16 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
                        dim3(tpb), 0, st, b->y, b->dy, b->total_points, uw0);
     hipLaunchKernelGGL(prep_template_kernel<Real>, dim3((unsigned)b->total_widths + 8), dim3(tpb),
                        0, st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->total_tmpl,
-                       L.tmpl_tail, tmpl2, tmpl1, tmplsq);
+                       L.tmpl_tail, tmpl2, tmpl1, tmplsq,
+                       L.tplen > 0 ? (uint32_t*)(ws + L.tmplh) : (uint32_t*)nullptr, L.tplen);
     if (L.n_chunked > 0)
         hipLaunchKernelGGL(chunk_template_kernel, dim3((unsigned)b->total_widths + 4), dim3(tpb), 0,
                            st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, b->skip_point,
-                           (float*)(ws + L.tmplc1), (float*)(ws + L.tmplc2),
-                           (float*)(ws + L.tzero), L.n_zero);
+                           (uint32_t*)(ws + L.tmplc), L.cplen, L.n_chunked, L.n_zero);
     hipLaunchKernelGGL(template_ss_kernel<Real>, dim3((unsigned)b->total_widths), dim3(kWave), 0,
                        st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, ss);
     hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
-                       b->y, b->dy, flat, wconst);
+                       b->y, b->dy, flat, wconst, (float2*)(ws + L.xsc));
     const int64_t ncount = 16 * L.n_chunks;
     hipLaunchKernelGGL(zero_u64s_kernel, dim3((unsigned)std::min<int64_t>(1024, (ncount + tpb - 1) / tpb)),
                        dim3(tpb), 0, st, counters, ncount);
@@ -234,9 +248,23 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.tmpl2 = tmpl2;
     p.tmpl1 = tmpl1;
     p.tmplsq = tmplsq;
-    p.tmplc1 = (const float*)(ws + L.tmplc1);
-    p.tmplc2 = (const float*)(ws + L.tmplc2);
-    p.tzero = (const float*)(ws + L.tzero);
+    p.tmplh = ws + L.tmplh;
+    p.tmplc = ws + L.tmplc;
+    p.tlo = (int32_t)(4 * L.tplen);
+    p.clo = (int32_t)(4 * L.cplen);
+    p.zoff = L.n_chunked;
+    p.xsc = (const float2*)(ws + L.xsc);
+    // curves with a common weight need the prefix sums D only if some width of the batch
+    // takes a path that reads them (the tiles compute their window means themselves)
+    p.d_constw = 0;
+    if (sizeof(Real) == 4) {
+        for (int32_t d = 1; d <= std::max<int32_t>(1, b->max_width); ++d) {
+            const int32_t q = (int32_t)((d > b->skip_point) ? (d / b->skip_point) : 1);
+            const bool toep = ctx->use_mfma && q < ctx->toep_q;
+            const bool diag = ctx->use_mfma && !toep && q >= 4 && d / q + 1 < kOnesCol;
+            if (!toep && !diag) p.d_constw = 1;
+        }
+    }
     p.ss = ss;
     p.chi2_flat = flat;
     p.wconst = wconst;
@@ -414,7 +442,9 @@ int64_t gtls_search_workspace_bytes(gtls_ctx* ctx, int32_t mode, const gtls_batc
     if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C) return fail(ctx, GTLS_E_MODE, "unknown mode");
     std::string why;
     if (!batch_ok(b, why)) return fail(ctx, GTLS_E_SHAPE, why);
-    return plan_search(ctx, mode, b).total;
+    const SearchLayout L = plan_search(ctx, mode, b);
+    if (!layout_ok(L, mode, why)) return fail(ctx, GTLS_E_SHAPE, why);
+    return L.total;
 }
 
 int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* b, void* workspace,
@@ -427,6 +457,7 @@ int gtls_search_batched(gtls_ctx* ctx, int32_t mode, const gtls_batch* b, void* 
     if (!workspace || !chi2 || !best_k || !best_s) return fail(ctx, GTLS_E_SHAPE, "NULL output or workspace");
     DeviceGuard guard(ctx->device);
     const SearchLayout L = plan_search(ctx, mode, b);
+    if (!layout_ok(L, mode, why)) return fail(ctx, GTLS_E_SHAPE, why);
     if (workspace_bytes < L.total)
         return fail(ctx, GTLS_E_WORKSPACE, "workspace too small: need " + std::to_string(L.total) +
                                                " bytes, got " + std::to_string(workspace_bytes));

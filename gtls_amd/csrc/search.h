@@ -53,10 +53,20 @@ struct SearchParams {
     const R2* tmpl2;        // per template sample (S, S^2)
     const Real* tmpl1;      // per template sample S
     const Real* tmplsq;     // per template sample S^2
-    // float32: chunked template rows of the matrix-core path (S, S^2) and a block of zeros
-    const float* tmplc1;
-    const float* tmplc2;
-    const float* tzero;
+    // float32 matrix-core operands in float16 pair planes (windows.h):
+    //   tmplh: template rows with kTmplPad zeros on both sides, planes of S (hi, hi) and
+    //          (lo, 0) pairs and of S^2 (hi, hi) and (lo, 0) pairs, each tlo bytes after
+    //          the previous one (the Toeplitz tiles);
+    //   tmplc: chunked template rows of the diagonal-sum tiles in the same four planes,
+    //          each clo bytes after the previous one and ending in a zero block at chunk
+    //          offset zoff
+    const char* tmplh;
+    const char* tmplc;
+    int32_t tlo, clo;
+    int64_t zoff;
+    const float2* xsc;      // per curve: power-of-two scales of the A-sum and weight planes
+    int32_t d_constw;       // float32: curves with a common weight need the prefix sums D
+                            // (some width takes a path that reads them)
     const double* ss;       // per width: sum of S^2 over the row
     const double* chi2_flat;  // per curve
     const double* wconst;   // per curve: the common weight 1/dy^2, or NaN if dy varies
@@ -108,14 +118,13 @@ __device__ __forceinline__ int xcd_id() {
 }
 
 // Gathers the samples of sorted positions o0 .. o0 + cnt - 1 (source sample src(j) for
-// j < cnt) in the search precision -- u alone with a common weight, else the planes
-// v = u w and w -- and writes the float64 running sum D of u, continuing from carry
-// (updated; equal in every thread). All threads call it; red_d is LDS of blockDim/64.
-template <class Real, class Src>
+// j < cnt) and hands each to put(position, u, w), and writes the float64 running sum D of
+// u (unless D is NULL), continuing from carry (updated; equal in every thread). All
+// threads call it; red_d is LDS of blockDim/64.
+template <class Real, class Src, class Put>
 __device__ __forceinline__ void gather_scan(const typename Pair<Real>::type* __restrict__ uw,
-                                            int64_t o0, int64_t cnt, Src src, bool constw,
-                                            Real* ub, Real* wpl, double* D, double& carry,
-                                            double* red_d) {
+                                            int64_t o0, int64_t cnt, Src src, Put put,
+                                            double* D, double& carry, double* red_d) {
     using R2 = typename Pair<Real>::type;
     const int lane = lane_id(), wave = wave_id();
     const int64_t chunk = (int64_t)kBlock * kScanPerThread;
@@ -133,16 +142,10 @@ __device__ __forceinline__ void gather_scan(const typename Pair<Real>::type* __r
 #pragma unroll
         for (int e = 0; e < kScanPerThread; ++e) {
             const int64_t j = wbase + (int64_t)e * kWave + lane;
-            Real uu = 0, ww = 0;
+            Real uu = 0;
             if (j < cnt) {
                 uu = g[e].x;
-                ww = g[e].y;
-                if (constw) {
-                    ub[o0 + j] = uu;
-                } else {
-                    ub[o0 + j] = uu * ww;
-                    wpl[o0 + j] = ww;
-                }
+                put(o0 + j, uu, g[e].y);
             }
             const double sc = wave_inclusive_scan((double)uu);
             u[e] = sc + run;
@@ -156,24 +159,79 @@ __device__ __forceinline__ void gather_scan(const typename Pair<Real>::type* __r
             total += red_d[w];
         }
         __syncthreads();
+        if (D) {
 #pragma unroll
-        for (int e = 0; e < kScanPerThread; ++e) {
-            const int64_t j = wbase + (int64_t)e * kWave + lane;
-            if (j < cnt) D[o0 + j] = u[e] + before;
+            for (int e = 0; e < kScanPerThread; ++e) {
+                const int64_t j = wbase + (int64_t)e * kWave + lane;
+                if (j < cnt) D[o0 + j] = u[e] + before;
+            }
         }
         carry = total;
     }
 }
 
+// Where prep_kernel puts the gathered samples of one period (windows.h for the layouts):
+// float64 -- u alone with a common weight, else the planes v = u w and w, at ub and wpl;
+// float32 -- the float16 (hi, lo) pair planes of sx u (common weight) or of sx v and sw w,
+// the first at hb and the second ps pairs after it.
+template <class Real> struct SampleSink;
+template <>
+struct SampleSink<double> {
+    double* ub;
+    double* wpl;
+    bool constw;
+    __device__ __forceinline__ void put(int64_t j, double u, double w) const {
+        if (constw) {
+            ub[j] = u;
+        } else {
+            ub[j] = u * w;
+            wpl[j] = w;
+        }
+    }
+    // position j repeats position jj (the wrap-around extension) / is zero
+    __device__ __forceinline__ void copy(int64_t j, int64_t jj) const {
+        ub[j] = ub[jj];
+        if (!constw) wpl[j] = wpl[jj];
+    }
+    __device__ __forceinline__ void zero(int64_t j) const {
+        ub[j] = 0.0;
+        if (!constw) wpl[j] = 0.0;
+    }
+};
+template <>
+struct SampleSink<float> {
+    uint32_t* hb;
+    int64_t ps;
+    float sx, sw;
+    bool constw;
+    __device__ __forceinline__ void put(int64_t j, float u, float w) const {
+        if (constw) {
+            hb[j] = split_pair(u * sx);
+        } else {
+            hb[j] = split_pair((u * w) * sx);
+            hb[j + ps] = split_pair(w * sw);
+        }
+    }
+    __device__ __forceinline__ void copy(int64_t j, int64_t jj) const {
+        hb[j] = hb[jj];
+        if (!constw) hb[j + ps] = hb[jj + ps];
+    }
+    __device__ __forceinline__ void zero(int64_t j) const {
+        hb[j] = 0u;
+        if (!constw) hb[j + ps] = 0u;
+    }
+};
+
 // gather_scan for one sort window of cnt <= kWinCap samples in float32 with a common
-// weight (X = u): each thread gathers kScanPerThread consecutive positions, sums them
-// serially, and one workgroup scan of the thread totals gives every position's running sum
-// of u (float64); u and D are staged in the window's LDS (su over the sample indices sidx,
-// sD over the phases) and copied out coalesced. All threads call it.
+// weight: each thread gathers kScanPerThread consecutive positions, sums them serially,
+// and one workgroup scan of the thread totals gives every position's running sum of u
+// (float64); u and D are staged in the window's LDS (su over the sample indices sidx, sD
+// over the phases) and copied out coalesced, u as the float16 (hi, lo) pairs of sx u at hb
+// and D unless it is NULL. All threads call it.
 __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw, int64_t o0,
                                                    int cnt, int32_t* sidx, double* sD,
-                                                   float* ub, double* D, double& carry,
-                                                   double* red_d) {
+                                                   uint32_t* hb, float sx, double* D,
+                                                   double& carry, double* red_d) {
     const int tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     const int p0 = tid * kScanPerThread;
     float* su = (float*)sidx;   // position p is read (as an index) before it is rewritten
@@ -209,9 +267,14 @@ __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw
     __syncthreads();
     // the windows kernel reads these only after the whole chunk is prepared: stored past
     // the caches, so that they do not evict the curves' times and sort scratch
-    for (int j = tid; j < cnt; j += kBlock) {
-        __builtin_nontemporal_store(su[j], &ub[o0 + j]);
-        __builtin_nontemporal_store(sD[j], &D[o0 + j]);
+    if (D) {
+        for (int j = tid; j < cnt; j += kBlock) {
+            __builtin_nontemporal_store(split_pair(su[j] * sx), &hb[o0 + j]);
+            __builtin_nontemporal_store(sD[j], &D[o0 + j]);
+        }
+    } else {
+        for (int j = tid; j < cnt; j += kBlock)
+            __builtin_nontemporal_store(split_pair(su[j] * sx), &hb[o0 + j]);
     }
     carry = total;
 }
@@ -277,54 +340,59 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
             // common weight only u is kept (X = u), otherwise the planes v = u w and w.
             const double wc = p.wconst[c];
             const bool constw = wc == wc;
-            Real* ub = (Real*)(p.xbuf + i * p.xstride);
-            Real* wpl = ub + p.xstride;
-            double* D = p.dbuf + i * p.xstride;
+            SampleSink<Real> sink;
+            if constexpr (sizeof(Real) == 8) {
+                sink.ub = (double*)(p.xbuf + i * p.xstride);
+                sink.wpl = sink.ub + p.xstride;
+            } else {
+                const float2 sc = p.xsc[c];
+                sink.hb = (uint32_t*)(p.xbuf + i * p.xstride);
+                sink.ps = p.xstride;
+                sink.sx = sc.x;
+                sink.sw = sc.y;
+            }
+            sink.constw = constw;
+            // float32 with a common weight: the running sums only if some width reads them
+            double* D = (sizeof(Real) == 4 && constw && !p.d_constw) ? nullptr
+                                                                    : p.dbuf + i * p.xstride;
             const R2* uw = p.uw0 + pt0;
             double carry = 0.0;
+            auto put = [&](int64_t j, Real u, Real w) { sink.put(j, u, w); };
             const bool windowed = wg_fold_sort_windows(
                 p.t + pt0, n, P, p.negative_time != 0, perm, hist, red_i, s_counts, s_phi,
                 s_idx, s_mids, s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                     if constexpr (sizeof(Real) == 4) {
                         if (constw) {
                             gather_scan_window((const float2*)uw, wstart, wcnt, s_idx, s_phi,
-                                               (float*)ub, D, carry, red_d);
+                                               sink.hb, sink.sx, D, carry, red_d);
                             return;
                         }
                     }
                     gather_scan<Real>(uw, wstart, wcnt, [&](int64_t j) { return s_idx[j]; },
-                                      constw, ub, wpl, D, carry, red_d);
+                                      put, D, carry, red_d);
                 });
             if (!windowed) {
                 // a bucket too large for LDS windows: the global form, then one gather pass
                 wg_fold_sort(p.t + pt0, n, P, p.negative_time != 0, key, perm, tmp_key, tmp_id,
                              lists, hist, red_i, s_counts);
-                gather_scan<Real>(uw, 0, n, [&](int64_t j) { return perm[j]; }, constw, ub, wpl,
-                                  D, carry, red_d);
+                gather_scan<Real>(uw, 0, n, [&](int64_t j) { return perm[j]; }, put, D, carry,
+                                  red_d);
             }
             __syncthreads();
             // the wrap-around extension: positions n .. np - 1 repeat positions 0 .. dmax - 1
             for (int64_t j = n + tid; j < np_; j += kBlock) {
                 const int64_t jj = (j - n) % n;
                 const int64_t laps = (j - n) / n + 1;
-                if (constw) {
-                    ub[j] = ub[jj];
-                } else {
-                    ub[j] = ub[jj];
-                    wpl[j] = wpl[jj];
-                }
-                D[j] = (double)laps * carry + D[jj];
+                sink.copy(j, jj);
+                if (D) D[j] = (double)laps * carry + D[jj];
             }
             // zeros past the sequence, up to where the window loops read (below
-            // N + d + q + 64 <= N + 2 dmax + 64 for the diagonal-sum tiles, below
-            // N + d + 15 q + 32 <= N + dmax + 377 for the Toeplitz tiles, q < 24): those
+            // N + d + q + 16 <= N + 2 dmax + 16 for the diagonal-sum tiles, below
+            // N + d + 15 q + 16 <= N + dmax + 361 for the Toeplitz tiles, q < 24): those
             // samples meet zero template entries
             const int64_t zend =
                 min(p.xstride, n + max(2 * (int64_t)dmax + 96, (int64_t)dmax + 416));
-            for (int64_t j = np_ + tid; j < zend; j += kBlock) {
-                ub[j] = (Real)0;
-                if (!constw) wpl[j] = (Real)0;
-            }
+            for (int64_t j = np_ + tid; j < zend; j += kBlock) sink.zero(j);
         }
         __syncthreads();
     }
@@ -412,26 +480,34 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
     const int32_t F = d / q;
     ArgMin w;
     if constexpr (sizeof(Real) == 4) {
+        // the float16 pair planes of this period (windows.h)
+        const char* base = (const char*)(p.xbuf + i * p.xstride);
+        const float2 sc = p.xsc[c];
+        HPlanes hp;
+        hp.X = base;
+        hp.W = base + 4 * p.xstride;
+        hp.inv_x = 1.f / sc.x;
+        hp.inv_w = 1.f / sc.y;
         if (p.use_mfma && q < p.toep_q) {
             const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
+            const char* T = p.tmplh + 4 * toff;
             if (constw)
-                w = toep_width<true>((const float*)ub, nullptr, (const float*)(p.tmpl1 + toff),
-                                     nullptr, D, a);
+                w = toep_width<true>(hp, T, nullptr, p.tlo, D, a);
             else
-                w = toep_width<false>((const float*)vw.v, (const float*)vw.w,
-                                      (const float*)(p.tmpl1 + toff),
-                                      (const float*)(p.tmplsq + toff), D, a);
+                w = toep_width<false>(hp, T, T + 2 * (int64_t)p.tlo, p.tlo, D, a);
             if (argmin_less(w, best)) best = w;
             return best;
         }
-        if (p.use_mfma && F + 1 <= 16) {
+        if (p.use_mfma && q >= 4 && F + 1 <= 16) {
             const int64_t co = uniform(chunk_offset(p.tmpl_off[wg], wg));
-            if (constw)
-                w = window_sums_mfma<true>((const float*)ub, nullptr, p.tmplc1 + co, nullptr,
-                                           p.tzero, D, a);
+            const char* C = p.tmplc + 4 * co;
+            const int32_t zrel = (int32_t)(4 * (p.zoff - co));
+            if (constw && F + 1 < kOnesCol)
+                w = mfma_width<true, true>(hp, C, nullptr, p.clo, zrel, D, a);
+            else if (constw)
+                w = mfma_width<true, false>(hp, C, nullptr, p.clo, zrel, D, a);
             else
-                w = window_sums_mfma<false>((const float*)vw.v, (const float*)vw.w,
-                                            p.tmplc1 + co, p.tmplc2 + co, p.tzero, D, a);
+                w = mfma_width<false, false>(hp, C, C + 2 * (int64_t)p.clo, p.clo, zrel, D, a);
             if (argmin_less(w, best)) best = w;
             return best;
         }
@@ -441,10 +517,22 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
     // within the buffer's padding of max_width samples
     int32_t R = (int32_t)min((int64_t)kMaxBlock, max((int64_t)1, (a.K + kWave - 1) / kWave));
     R = uniform(min(R, 1 + p.max_width / q));
-    if (constw)
-        w = valu_width<Real, true>(ub, p.tmpl1 + toff, D, a, R);
-    else
-        w = valu_width<Real, false>(vw, (const Real*)(p.tmpl2 + toff), D, a, R);
+    if constexpr (sizeof(Real) == 4) {
+        const char* base = (const char*)(p.xbuf + i * p.xstride);
+        const float2 sc = p.xsc[c];
+        const HalfX hx{(const uint32_t*)base, 1.f / sc.x};
+        if (constw) {
+            w = valu_width<Real, true>(hx, p.tmpl1 + toff, D, a, R);
+        } else {
+            const HalfX hw{(const uint32_t*)base + p.xstride, 1.f / sc.y};
+            w = valu_width<Real, false>(HalfVW{hx, hw}, (const Real*)(p.tmpl2 + toff), D, a, R);
+        }
+    } else {
+        if (constw)
+            w = valu_width<Real, true>(ub, p.tmpl1 + toff, D, a, R);
+        else
+            w = valu_width<Real, false>(vw, (const Real*)(p.tmpl2 + toff), D, a, R);
+    }
     if (argmin_less(w, best)) best = w;
     return best;
 }
@@ -568,25 +656,47 @@ __global__ void prep_points_kernel(const double* y, const double* dy, int64_t n,
     uw[i] = o;
 }
 
+// Template value sv at pair index i of the four float16 pair planes from th (plen pairs
+// each): S (hi, hi), S (lo, 0), S^2 (hi, hi), S^2 (lo, 0), S rounded to float32 first.
+__device__ __forceinline__ void put_template_pairs(uint32_t* th, int64_t plen, int64_t i,
+                                                   double sv) {
+    const float s1 = (float)sv, s2 = (float)(sv * sv);
+    const _Float16 h1 = (_Float16)s1, h2 = (_Float16)s2;
+    th[i] = f16_pair(h1, h1);
+    th[i + plen] = f16_pair((_Float16)(s1 - (float)h1), (_Float16)0.f);
+    th[i + 2 * plen] = f16_pair(h2, h2);
+    th[i + 3 * plen] = f16_pair((_Float16)(s2 - (float)h2), (_Float16)0.f);
+}
+
 // Search-precision template rows, (S, S^2) pairs, S alone and S^2 alone, each row surrounded by
 // kTmplPad zeros: row w starts at tmpl_off[w] + 2 kTmplPad w + kTmplPad. One block per row.
+// Float32: also the float16 pair planes of S and S^2 (S rounded to float32 first) for the
+// Toeplitz tiles, five planes of plen pairs from th: S (hi, hi), S (lo, 0), S^2 (hi, hi),
+// S^2 (lo, 0), and (1, 1) over the row's d samples (zero in the padding).
 template <class Real>
 __global__ void prep_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
                                      int64_t nw, int64_t total_tmpl, int64_t tail,
-                                     typename Pair<Real>::type* out, Real* out1, Real* outsq) {
+                                     typename Pair<Real>::type* out, Real* out1, Real* outsq,
+                                     uint32_t* th, int64_t plen) {
+    auto put = [&](int64_t i, double sv, bool one) {
+        typename Pair<Real>::type o;
+        o.x = (Real)sv;
+        o.y = (Real)(sv * sv);
+        out[i] = o;
+        out1[i] = (Real)sv;
+        outsq[i] = (Real)(sv * sv);
+        if (th) {
+            put_template_pairs(th, plen, i, sv);
+            th[i + 4 * plen] = one ? f16_pair((_Float16)1.f, (_Float16)1.f) : 0u;
+        }
+    };
     const int64_t w = blockIdx.x;
     if (w >= nw) {
-        // the zeroed tail after the last row (over-reads of the matrix-core path)
+        // the zeroed tail after the last row (over-reads of the window loops)
         const int64_t end = total_tmpl + 2 * kTmplPad * nw;
         for (int64_t i = end + (int64_t)(w - nw) * blockDim.x + threadIdx.x;
-             i < end + tail; i += (int64_t)(gridDim.x - nw) * blockDim.x) {
-            typename Pair<Real>::type o;
-            o.x = (Real)0;
-            o.y = (Real)0;
-            out[i] = o;
-            out1[i] = (Real)0;
-            outsq[i] = (Real)0;
-        }
+             i < end + tail; i += (int64_t)(gridDim.x - nw) * blockDim.x)
+            put(i, 0.0, false);
         return;
     }
     const int32_t d = max(widths[w], 0);
@@ -594,27 +704,25 @@ __global__ void prep_template_kernel(const double* S, const int64_t* off, const 
     const int64_t dst = off[w] + 2 * kTmplPad * w;   // start of the padded row
     for (int32_t i = threadIdx.x; i < d + 2 * kTmplPad; i += blockDim.x) {
         const int32_t j = i - kTmplPad;
-        const double s = (j >= 0 && j < d) ? S[src + j] : 0.0;
-        typename Pair<Real>::type o;
-        o.x = (Real)s;
-        o.y = (Real)(s * s);
-        out[dst + i] = o;
-        out1[dst + i] = (Real)s;
-        outsq[dst + i] = (Real)(s * s);
+        const bool in = j >= 0 && j < d;
+        put(dst + i, in ? S[src + j] : 0.0, in);
     }
 }
 
-// Chunked template rows of the matrix-core path (float32): one block per width; widths the
-// matrix-core path does not take (q < 4 or more than 16 chunks) get no row. The extra
-// blocks past the last width clear the zero block (nz floats).
+// Chunked template rows of the diagonal-sum tiles (float32) in the four pair planes of
+// plen pairs from c (as put_template_pairs), each ending in a zero block of nz pairs at
+// zoff; after the F + 1 template chunks, the ones chunks F + 1 (b < q) and F + 2
+// (b < d - F q), as (1, 1) and (0, 0) pairs. One block per width; widths those tiles do
+// not take (q < 4 or more than 16 chunks) get no row. The extra blocks past the last width
+// clear the zero blocks.
 __global__ void chunk_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
-                                      int64_t nw, int64_t skip_point, float* c1, float* c2,
-                                      float* zero, int64_t nz) {
+                                      int64_t nw, int64_t skip_point, uint32_t* c, int64_t plen,
+                                      int64_t zoff, int64_t nz) {
     const int64_t w = blockIdx.x;
     if (w >= nw) {
         for (int64_t i = (int64_t)(w - nw) * blockDim.x + threadIdx.x; i < nz;
              i += (int64_t)(gridDim.x - nw) * blockDim.x)
-            zero[i] = 0.f;
+            put_template_pairs(c, plen, zoff + i, 0.0);
         return;
     }
     const int32_t d = widths[w];
@@ -624,13 +732,17 @@ __global__ void chunk_template_kernel(const double* S, const int64_t* off, const
     if (q < 4 || F + 1 > 16) return;
     const int32_t qp = chunk_pad(q);
     const int64_t co = chunk_offset(off[w], w);
-    const int32_t total = (F + 1) * qp;
+    const int32_t total = (F + 3) * qp;
+    const uint32_t one = f16_pair((_Float16)1.f, (_Float16)1.f);
     for (int32_t i = threadIdx.x; i < total; i += blockDim.x) {
-        const int32_t c = i / qp, b = i - c * qp;
-        const int32_t len = c < F ? q : rem;
-        const double s = b < len ? S[off[w] + (int64_t)c * q + b] : 0.0;
-        c1[co + i] = (float)s;
-        c2[co + i] = (float)(s * s);
+        const int32_t ch = i / qp, b = i - ch * qp;
+        if (ch <= F) {
+            const int32_t len = ch < F ? q : rem;
+            put_template_pairs(c, plen, co + i, b < len ? S[off[w] + (int64_t)ch * q + b] : 0.0);
+        } else {
+            const bool in = b < (ch == F + 1 ? q : rem);
+            for (int k = 0; k < 4; ++k) c[co + i + k * plen] = (in && k % 2 == 0) ? one : 0u;
+        }
     }
 }
 
@@ -652,12 +764,17 @@ __global__ __launch_bounds__(kWave) void template_ss_kernel(const double* S, con
     if (threadIdx.x == 0) ss[k] = acc;
 }
 
-// Flat-model chi2 of each curve, sum (1 - y)^2 / dy^2, float64, and the curve's common
-// weight 1/dy^2 when every dy is the same value (NaN otherwise); one workgroup per curve.
+// Flat-model chi2 of each curve, sum (1 - y)^2 / dy^2, float64, the curve's common
+// weight 1/dy^2 when every dy is the same value (NaN otherwise), and the power-of-two
+// scales of the float32 search's float16 planes (f16_scale of the largest |u| with a
+// common weight, else of the largest |u w|, and of the largest w; u and w rounded to
+// float32 as the search gathers them); one workgroup per curve.
 __global__ __launch_bounds__(kBlock) void chi2_flat_kernel(const int64_t* pt_off,
                                                            const double* y, const double* dy,
-                                                           double* out, double* wconst) {
+                                                           double* out, double* wconst,
+                                                           float2* xsc) {
     __shared__ double red[kBlock / kWave];
+    __shared__ float red_u[kBlock / kWave], red_v[kBlock / kWave], red_w[kBlock / kWave];
     __shared__ int differ;
     const int c = blockIdx.x;
     const int64_t a = pt_off[c], b = pt_off[c + 1];
@@ -665,6 +782,7 @@ __global__ __launch_bounds__(kBlock) void chi2_flat_kernel(const int64_t* pt_off
     __syncthreads();
     const double e0 = dy[a];
     double s = 0.0;
+    float mu = 0.f, mv = 0.f, mw = 0.f;
     int dif = 0;
     for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
         const double u = 1.0 - y[i];
@@ -672,16 +790,38 @@ __global__ __launch_bounds__(kBlock) void chi2_flat_kernel(const int64_t* pt_off
         const double w = 1.0 / (e * e);
         s += u * u * w;
         dif |= (e != e0);
+        const float uf = (float)u, wf = (float)w;
+        mu = fmaxf(mu, fabsf(uf));
+        mv = fmaxf(mv, fabsf(uf * wf));
+        mw = fmaxf(mw, wf);
     }
     if (dif) atomicOr(&differ, 1);
     s = wave_sum(s);
-    if (lane_id() == 0) red[wave_id()] = s;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        mu = fmaxf(mu, __shfl_xor(mu, off, kWave));
+        mv = fmaxf(mv, __shfl_xor(mv, off, kWave));
+        mw = fmaxf(mw, __shfl_xor(mw, off, kWave));
+    }
+    if (lane_id() == 0) {
+        red[wave_id()] = s;
+        red_u[wave_id()] = mu;
+        red_v[wave_id()] = mv;
+        red_w[wave_id()] = mw;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         double tot = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) tot += red[w];
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            tot += red[w];
+            mu = fmaxf(mu, red_u[w]);
+            mv = fmaxf(mv, red_v[w]);
+            mw = fmaxf(mw, red_w[w]);
+        }
         out[c] = tot;
         wconst[c] = differ ? __builtin_nan("") : 1.0 / (e0 * e0);
+        xsc[c] = differ ? make_float2(f16_scale(mv), f16_scale(mw))
+                        : make_float2(f16_scale(mu), 1.f);
     }
 }
 

@@ -184,41 +184,86 @@ __device__ __forceinline__ void window_sums_q1(XS X, const Real* Tg, int32_t d, 
 namespace gtls {
 
 // ---------------------------------------------------------------------------------------
+// Float32 search operands in float16 high/low pairs. A value x (scaled by a power of two
+// so that |x| < 2^14) is stored as the pair (hi, lo) = (f16(x), f16(x - hi)) in one
+// 32-bit word; hi + lo holds x to about 2^-22 relative. Sample planes are arrays of such
+// pairs. A template value s is stored in two pair planes, (hi(s), hi(s)) and (lo(s), 0),
+// so that with the K dimension running over (sample, half) the two products
+//     sum_i hi(x_i) hi(s_i) + lo(x_i) hi(s_i)   and   sum_i hi(x_i) lo(s_i)
+// are one v_mfma_f32_16x16x32_f16 each per 16 samples (the dropped lo lo term is ~2^-22
+// of the product). Every operand is 4 consecutive pairs (16 bytes) starting at a pair
+// boundary, so every load is dword-aligned, and the window sums keep float32 accuracy at
+// 2 MFMAs of 16 x 16 x 32 per 16 samples instead of 4 of 16 x 16 x 4.
+// ---------------------------------------------------------------------------------------
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t f16_pair(_Float16 a, _Float16 b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+// The (hi, lo) pair of x.
+__device__ __forceinline__ uint32_t split_pair(float x) {
+    const _Float16 hi = (_Float16)x;
+    return f16_pair(hi, (_Float16)(x - (float)hi));
+}
+__device__ __forceinline__ float pair_value(uint32_t p) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(p & 0xffffu)) +
+           (float)__builtin_bit_cast(_Float16, (uint16_t)(p >> 16));
+}
+
+// Power-of-two scale s with max_abs * s < 2^14 (1 for max_abs = 0 or not finite).
+__host__ __device__ inline float f16_scale(float max_abs) {
+    if (!(max_abs > 0.f) || !(max_abs < 3.0e38f)) return 1.f;
+    int e = 0;
+    (void)frexpf(max_abs, &e);        // max_abs < 2^e
+    e = e < -100 ? -100 : (e > 100 ? 100 : e);
+    return ldexpf(1.f, 14 - e);
+}
+
+// Samples of the float32 VALU windows read from a pair plane: (hi + lo) / scale.
+struct HalfX {
+    const uint32_t* p;
+    float inv;       // 1 / scale
+    __device__ __forceinline__ float operator[](int64_t i) const { return pair_value(p[i]) * inv; }
+    __device__ __forceinline__ HalfX operator+(int64_t o) const { return HalfX{p + o, inv}; }
+};
+// ... and the (v, w) pairs of the weighted path.
+struct HalfVW {
+    HalfX v, w;
+    __device__ __forceinline__ float2 operator[](int64_t i) const {
+        return make_float2(v[i], w[i]);
+    }
+    __device__ __forceinline__ HalfVW operator+(int64_t o) const { return HalfVW{v + o, w + o}; }
+};
+
+// ---------------------------------------------------------------------------------------
 // Matrix-core window sums (float32 search). For stride q and width d = F q + rem, the
 // window of candidate k covers data blocks k .. k + F (block j = samples jq .. jq + q - 1):
 //     G(j, c) = sum_{b<q} S[c q + b] X[j q + b],      A(k) = sum_{c=0}^{F} G(k + c, c),
 // (chunk F holds the rem leftover template samples). G is the product of the block matrix
 // (rows j, q columns) and the chunk matrix (F + 1 <= 16 columns); it runs on
-// v_mfma_f32_16x16x4f32 (operand lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result
-// register v of lane l holds C[4 (l / 16) + v][l % 16]).
+// v_mfma_f32_16x16x32_f16 over the pair planes (operand lanes: A[l % 16][8 (l / 16) + e],
+// B[8 (l / 16) + e][l % 16], e < 8, i.e. the 4 pairs of samples 4 (l / 16) .. + 3 of a
+// K group of 16 samples; result register v of lane l holds C[4 (l / 16) + v][l % 16]).
 //
 // A super-tile is 4 tiles of 16 G rows (rows 64 T .. 64 T + 63); the 4 tiles share the
 // chunk operand. Finished rows go to an 80-row LDS ring per wavefront (64 new rows plus
 // the F <= 15 rows of the previous super-tile that its last candidates still need); then
 // each lane takes the diagonal sum of one candidate, k = 64 T - F + lane.
 //
-// The K (b) loop runs in groups of 4 M samples = M MFMA steps; within a group, step s uses
-// sample b0 + M (l / 16) + s in K slot l / 16, so each lane's operands for a group are M
-// consecutive floats. The loads of the next group are issued before the MFMAs of the
-// current one, and the first group of the next sweep (the B plane, or the next
-// super-tile) before the last MFMAs of this one, so its latency passes during the
-// diagonal sums.
+// The loads of the next K group are issued before the MFMAs of the current one, and the
+// first group of the next sweep (the B plane, or the next super-tile) before the last
+// MFMAs of this one, so its latency passes during the diagonal sums.
 // ---------------------------------------------------------------------------------------
 typedef float mfma_v4 __attribute__((ext_vector_type(4)));
 constexpr int kRingRows = 80;     // 64 rows of a super-tile + up to 15 look-ahead rows
 constexpr int kRingPitch = 84;    // column-major: column c at c * kRingPitch (16-byte rows of 4)
 constexpr int kRingFloats = 16 * kRingPitch;
+constexpr int kKGroup = 16;       // samples per K group (one 16x16x32 MFMA step over pairs)
 
-
-template <int M>
-struct FV {
-    float v[M];
-};
-
-// M consecutive floats (one 4 M-byte load).
-template <int M>
-__device__ __forceinline__ FV<M> ldm(const float* p) {
-    FV<M> r;
+// 4 consecutive pairs (one 16-byte load).
+__device__ __forceinline__ h8 ld_h8(const char* p) {
+    h8 r;
     __builtin_memcpy(&r, p, sizeof(r));
     return r;
 }
@@ -228,144 +273,162 @@ __device__ __forceinline__ int ring_wrap(int r) {
     return r >= kRingRows ? r - kRingRows : r;
 }
 
-
-// K-group size of the matrix-core path: 4 M samples per group, M MFMA steps per tile. Wide
-// strides take long groups so that the MFMAs of one group cover the latency of the next
-// group's loads -- unless 32-sample groups would pad q further than 16-sample ones;
-// short strides take short groups so that little of a group is padding.
-__host__ __device__ inline int32_t mfma_m(int32_t q) {
-    if (q >= 48) return (q + 31) / 32 * 32 == (q + 15) / 16 * 16 ? 8 : 4;
-    return q >= 16 ? 4 : 2;
-}
-
 // Chunked template rows of the matrix-core path: for width w with stride q, F = d / q and
-// q_pad = q rounded up to whole K groups (4 mfma_m(q) samples), chunk c <= F sits at
-// chunk_offset(w) + c q_pad, holds S[c q + b] for b < its length (q, or d - F q for
-// c = F) and zeros up to q_pad. Columns c > F read a block of zeros instead.
-// (F + 1) q_pad <= 2 d + 1024 for F <= 15, so the offsets below leave room for every row.
+// q_pad = q rounded up to whole K groups, chunk c <= F sits at chunk_offset(w) + c q_pad
+// (in pairs of each plane), holds S[c q + b] for b < its length (q, or d - F q for c = F)
+// and zeros up to q_pad; chunks F + 1 and F + 2 hold ones for b < q and b < d - F q (the
+// window sums of the samples themselves, in tile columns 14 and 15 when F <= 13). Other
+// columns read the zero block that follows the rows of each plane.
+// (F + 3) q_pad <= 4 d + 270, so the offsets below leave room for every row.
+constexpr int kOnesCol = 14;     // tile column of the ones chunk (F + 1); the rem ones in 15
 __host__ __device__ inline int64_t chunk_offset(int64_t tmpl_off, int64_t w) {
-    return 2 * tmpl_off + 1024 * w;
+    return 4 * tmpl_off + 1024 * w;
 }
 __host__ __device__ inline int32_t chunk_pad(int32_t q) {
-    const int32_t g = 4 * mfma_m(q);
-    return (q + g - 1) / g * g;
+    return (q + kKGroup - 1) / kKGroup * kKGroup;
 }
 
-// Operand addresses of one sweep (the 64 G rows of super-tile st for one sample plane X
-// and chunked template C): the lane's A pointers (4 tiles) and B pointer, and the group
-// count. A operand X[row q + b]: samples b >= q meet zero template entries and rows past
-// the data read block 0, so the sample plane needs only finite values (the gather
-// zero-fills its tail); B operand C[c q_pad + b], columns c > F read the zero block Z.
-template <int M>
-struct GSweep {
-    const float* X;      // sample plane (wave-uniform)
-    const float* C;      // template rows (wave-uniform)
-    uint32_t xo[4];      // lane's A operand byte offsets from X, one per tile
-    int32_t co;          // lane's B operand byte offset from C
+// Operand byte offsets of one sweep over NT tiles: the sample pair plane X and the first
+// template pair plane C ((hi, hi) pairs; the (lo, 0) pairs sit clo bytes further), the
+// lane's A offsets (one per tile) and B offset.
+template <int NT>
+struct HSweep {
+    const char* X;       // sample pair plane (wave-uniform)
+    const char* C;       // template (hi, hi) pair plane (wave-uniform)
+    uint32_t xo[NT];     // lane's A operand byte offsets, one per tile
+    int32_t co;          // lane's B operand byte offset
 };
 
-template <int M>
-__device__ __forceinline__ GSweep<M> g_sweep(const float* __restrict__ X,
-                                             const float* __restrict__ C,
-                                             const float* __restrict__ Z, int32_t q, int32_t F,
-                                             int64_t rows, int64_t st) {
+// The diagonal-sum sweep of super-tile st (64 G rows): A operand X[row q + b]: samples
+// b >= q meet zero template entries and rows past the data read block 0, so the sample
+// plane needs only finite values (the gather zero-fills its tail); B operand
+// C[c q_pad + b] for columns c <= F, the ones chunks F + 1 and F + 2 in columns 14 and 15
+// when `ones` (F <= 13), and the zero block (zrel bytes from C) in the other columns.
+__device__ __forceinline__ HSweep<4> h_sweep(const char* __restrict__ X, const char* __restrict__ C,
+                                             int32_t zrel, int32_t q, int32_t F, int64_t rows,
+                                             int64_t st, bool ones = false) {
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
-    GSweep<M> w;
+    HSweep<4> w;
     w.X = X;
     w.C = C;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int64_t row = 64 * st + 16 * t + col;
-        w.xo[t] = (uint32_t)(4 * ((row < rows ? row * q : 0) + M * kk));
+        w.xo[t] = (uint32_t)(4 * ((row < rows ? row * q : 0) + 4 * kk));
     }
-    w.co = (int32_t)(4 * ((col <= F) ? (int64_t)col * chunk_pad(q) + M * kk
-                                     : (int64_t)(Z - C) + M * kk));
+    const int32_t ch = col <= F ? col : (ones && col >= kOnesCol ? F + 1 + (col - kOnesCol) : -1);
+    w.co = (ch >= 0) ? 4 * (ch * chunk_pad(q) + 4 * kk) : zrel + 16 * kk;
     return w;
 }
 
-// One K group's operands of a sweep (M MFMA steps for each of the 4 tiles).
-template <int M>
-struct GSet {
-    FV<M> a[4];
-    FV<M> b;
+// One K group's operands of a sweep: the sample pairs of the NT tiles and the two
+// template pair operands.
+// With Ones, also a third template operand: (1, 1) pairs where the lane's taps fall inside
+// the window of its column (a row of d ones shifted like the template: the window sums of
+// the samples, Toeplitz tiles), formed in registers from the lane's template tap
+// w.co / 4 + 16 g and the width d.
+template <int NT, bool Ones = false>
+struct HSet {
+    h8 a[NT];
+    h8 b1, b2;
+    h8 b3[Ones ? 1 : 0];
 };
 
-template <int M>
-__device__ __forceinline__ void g_load(const GSweep<M>& w, int32_t g, GSet<M>& o) {
-    const uint32_t off = 16 * M * g;
-    const char* x = (const char*)w.X;
-    const char* c = (const char*)w.C;
+template <int NT, bool Ones>
+__device__ __forceinline__ void h_load(const HSweep<NT>& w, int32_t g, int32_t clo,
+                                       HSet<NT, Ones>& o, int32_t d = 0) {
+    const int32_t off = 4 * kKGroup * g;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) o.a[t] = ldm<M>((const float*)(x + (w.xo[t] + off)));
-    o.b = ldm<M>((const float*)(c + (int64_t)(w.co + (int32_t)off)));
+    for (int t = 0; t < NT; ++t) o.a[t] = ld_h8(w.X + (w.xo[t] + (uint32_t)off));
+    const char* c = w.C + (int64_t)(w.co + off);
+    o.b1 = ld_h8(c);
+    o.b2 = ld_h8(c + clo);
+    if constexpr (Ones) {
+        const int32_t tap = (w.co + off) / 4;   // exact: w.co and off are multiples of 4
+        uint32_t v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            v[e] = ((uint32_t)(tap + e) < (uint32_t)d) ? 0x3C003C00u : 0u;   // (1, 1)
+        __builtin_memcpy(&o.b3[0], v, sizeof(v));
+    }
 }
 
-// K groups of a sweep over `len` taps (4 M samples per group; taps past `len` meet zero
-// template entries).
-__device__ __forceinline__ int32_t k_groups(int32_t len, int M) {
-    return (len + 4 * M - 1) / (4 * M);
+// K groups of a sweep over `len` taps (taps past `len` meet zero template entries).
+__device__ __forceinline__ int32_t k_groups(int32_t len) {
+    return (len + kKGroup - 1) / kKGroup;
+}
+
+// The two pair products of one K group into the NT tiles' accumulators (each pass over
+// the tiles before the next, so that consecutive MFMAs are independent); with Ones, the
+// sample sums into accw.
+template <int NT, bool Ones>
+__device__ __forceinline__ void h_mma(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
+                                      const HSet<NT, Ones>& o) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(o.a[t], o.b1, acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(o.a[t], o.b2, acc[t], 0, 0, 0);
+    if constexpr (Ones) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            accw[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(o.a[t], o.b3[0], accw[t], 0, 0, 0);
+    }
 }
 
 // One sweep into acc (zeroed here): `cur` with its group 0 already in s0; on return s0
 // holds group 0 of the sweep `nxt`, whose loads stay in flight through whatever the caller
 // does next. Two explicit operand sets with straight-line pairs of groups: each MFMA
 // block waits only for its own set while the other set's loads are in flight (the
-// scheduling barriers keep each load block ahead of the MFMA block before it). Within a
-// group, step i takes sample 4 M g + M kk + i in K slot kk, so each lane's operands are M
-// consecutive floats.
-template <int M>
-__device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[4], const GSweep<M>& cur,
-                                          const GSweep<M>& nxt, int32_t ng, GSet<M>& s0,
-                                          GSet<M>& s1) {
+// scheduling barriers keep each load block ahead of the MFMA block before it).
+template <int NT, bool Ones>
+__device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
+                                          const HSweep<NT>& cur, const HSweep<NT>& nxt,
+                                          int32_t ng, int32_t clo, HSet<NT, Ones>& s0,
+                                          HSet<NT, Ones>& s1, int32_t d = 0) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
-    auto mma = [&](const GSet<M>& o) {
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[t].v[i], o.b.v[i], acc[t],
-                                                             0, 0, 0);
-        }
-    };
+    for (int t = 0; t < NT; ++t) {
+        acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (Ones) accw[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+    }
     int32_t g = 0;
     for (; g + 1 < ng; g += 2) {
-        g_load(cur, g + 1, s1);
+        h_load(cur, g + 1, clo, s1, d);
         __builtin_amdgcn_sched_barrier(0);
-        mma(s0);
+        h_mma(acc, accw, s0);
         __builtin_amdgcn_sched_barrier(0);
         // group g + 2 of this sweep, or group 0 of the next one (addresses selected, so
         // the loads are unconditional)
         const bool more = g + 2 < ng;
-        GSweep<M> w;
+        HSweep<NT> w;
         w.X = more ? cur.X : nxt.X;
         w.C = more ? cur.C : nxt.C;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) w.xo[t] = more ? cur.xo[t] : nxt.xo[t];
+        for (int t = 0; t < NT; ++t) w.xo[t] = more ? cur.xo[t] : nxt.xo[t];
         w.co = more ? cur.co : nxt.co;
-        g_load(w, more ? g + 2 : 0, s0);
+        h_load(w, more ? g + 2 : 0, clo, s0, d);
         __builtin_amdgcn_sched_barrier(0);
-        mma(s1);
+        h_mma(acc, accw, s1);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (g < ng) {   // an odd group count: the last group is in s0
-        mma(s0);
+        h_mma(acc, accw, s0);
         __builtin_amdgcn_sched_barrier(0);
-        g_load(nxt, 0, s0);
+        h_load(nxt, 0, clo, s0, d);
     }
 }
 
 // The sweep `cur` of a super-tile, written to the ring (see run_sweep).
-template <int M>
-__device__ __forceinline__ void super_tile_rows(const GSweep<M>& cur, const GSweep<M>& nxt,
-                                                int32_t ng, GSet<M>& s0, int ring0,
-                                                float* ring) {
+__device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSweep<4>& nxt,
+                                                int32_t ng, int32_t clo, HSet<4>& s0,
+                                                int ring0, float* ring) {
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
-    mfma_v4 acc[4];
-    GSet<M> s1;
-    run_sweep<M>(acc, cur, nxt, ng, s0, s1);
+    mfma_v4 acc[4], unused[4];
+    HSet<4> s1;
+    run_sweep<4, false>(acc, unused, cur, nxt, ng, clo, s0, s1);
     // rows 16 t + 4 kk .. + 3 of column col: one 16-byte store (ring0 is a multiple of 16
     // and kRingRows of 4, so the four rows never wrap apart)
 #pragma unroll
@@ -379,19 +442,28 @@ __device__ __forceinline__ void super_tile_rows(const GSweep<M>& cur, const GSwe
 // The window rings of the four wavefronts of a workgroup (A sums, then B sums).
 __shared__ __attribute__((aligned(16))) float g_rings[2 * 4 * kRingFloats];
 
+// Sample pair planes of one (period, width) item of the matrix-core paths: the A-sum
+// samples X (u with a common weight, v = u w otherwise) and the weights W (weighted
+// path), and 1 / scale of each.
+struct HPlanes {
+    const char* X;
+    const char* W;
+    float inv_x, inv_w;
+};
+
 // Every candidate of one (period, width) item on the matrix cores, returning the lane's
-// first minimum of the statistic. X, C: the sample plane and chunked template of the A
-// sums (u with S when ConstW, v = u w with S otherwise); X2, C2: those of the B sums
-// (w with S^2; unused when ConstW); Z: zeros (at least q_pad floats). Reads of X stay
-// below N + d + q + 64 (zero-filled past the gathered samples). D: prefix sums of u
-// (float64) for the window means.
-template <bool ConstW, int M>
-__device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
-                                          const float* __restrict__ X2,
-                                          const float* __restrict__ C,
-                                          const float* __restrict__ C2,
-                                          const float* __restrict__ Z,
-                                          const double* __restrict__ D, WidthArgs a) {
+// first minimum of the statistic. C, C2: chunked template pair planes of S and S^2
+// ((hi, hi) pairs; (lo, 0) pairs clo bytes further; their zero blocks zrel bytes from C
+// and C2). Reads of X stay below N + d + q + 16 (zero-filled past the gathered samples).
+// The window means: with TileMean (common weight, F <= 13), from the ones columns of the
+// same tiles, the window sum of u = sum_{c<F} G(k + c, 14) + G(k + F, 15) in float32;
+// otherwise from D, the float64 prefix sums of u.
+template <bool ConstW, bool TileMean>
+__device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __restrict__ C,
+                                             const char* __restrict__ C2, int32_t clo,
+                                             int32_t zrel, const double* __restrict__ D,
+                                             WidthArgs a) {
+    static_assert(ConstW || !TileMean, "tile means need the samples u");
     const int lane = lane_id();
     const int wave = uniform((int32_t)wave_id());
     float* ringA = g_rings + wave * kRingFloats;
@@ -402,34 +474,38 @@ __device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
     const int64_t rows = K + F;
     const int64_t nsuper = (rows + 63) / 64;
     ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
-    const int32_t ng = k_groups(q, M);
+    const int32_t ng = chunk_pad(q) / kKGroup;
     int ring0 = 0;   // ring row of G row 64 T
     // sweeps in order (A plane, then B plane, of each super-tile); the first group of each
     // sweep is loaded during the one before (past the last super-tile: row block 0 again)
-    GSet<M> ops;
-    g_load(g_sweep<M>(X, C, Z, q, F, rows, 0), 0, ops);
+    HSet<4> ops;
+    h_load(h_sweep(P.X, C, zrel, q, F, rows, 0, TileMean), 0, clo, ops);
     for (int64_t st = 0; st < nsuper; ++st) {
-        // the window-mean prefix sums of this super-tile's candidates, requested early
         const int64_t kc = 64 * st - F + lane;
         const bool cand = kc >= 0 && kc < K;
+        // the window-mean prefix sums of this super-tile's candidates, requested early
         double dlo = 0.0, dhi = 0.0;
-        if (cand) {
-            const int64_t s = kc * q;
-            dhi = D[s + d - 1];
-            dlo = s ? D[s - 1] : 0.0;
+        if constexpr (!TileMean) {
+            if (cand) {
+                const int64_t s = kc * q;
+                dhi = D[s + d - 1];
+                dlo = s ? D[s - 1] : 0.0;
+            }
         }
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
-        const GSweep<M> wa = g_sweep<M>(X, C, Z, q, F, rows, st);
+        const HSweep<4> wa = h_sweep(P.X, C, zrel, q, F, rows, st, TileMean);
         if constexpr (ConstW) {
-            super_tile_rows<M>(wa, g_sweep<M>(X, C, Z, q, F, rows, stn), ng, ops, ring0, ringA);
+            super_tile_rows(wa, h_sweep(P.X, C, zrel, q, F, rows, stn, TileMean), ng, clo, ops,
+                            ring0, ringA);
         } else {
-            const GSweep<M> wb = g_sweep<M>(X2, C2, Z, q, F, rows, st);
-            super_tile_rows<M>(wa, wb, ng, ops, ring0, ringA);
-            super_tile_rows<M>(wb, g_sweep<M>(X, C, Z, q, F, rows, stn), ng, ops, ring0, ringB);
+            const HSweep<4> wb = h_sweep(P.W, C2, zrel, q, F, rows, st);
+            super_tile_rows(wa, wb, ng, clo, ops, ring0, ringA);
+            super_tile_rows(wb, h_sweep(P.X, C, zrel, q, F, rows, stn), ng, clo, ops, ring0,
+                            ringB);
         }
         wave_sync_lds();
-        // diagonal sums: all 16 reads issue before the first add, columns past F masked
-        float va[16], vb[16];
+        // diagonal sums: all reads issue before the first add, columns past F masked
+        float va[16], vb[16], vu[15];
         const int r0 = ring0 - F + lane;
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
@@ -437,16 +513,30 @@ __device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
             va[c] = ringA[c * kRingPitch + r];
             if constexpr (!ConstW) vb[c] = ringB[c * kRingPitch + r];
         }
-        float sa = 0.f, sb = 0.f;
+        if constexpr (TileMean) {
+            // block sums (column 14) of blocks k .. k + F - 1, the rem sum (15) of block k + F
+#pragma unroll
+            for (int c = 0; c < 14; ++c)
+                vu[c] = ringA[kOnesCol * kRingPitch + ring_wrap(r0 + min(c, F))];
+            vu[14] = ringA[(kOnesCol + 1) * kRingPitch + ring_wrap(r0 + F)];
+        }
+        float sa = 0.f, sb = 0.f, su = 0.f;
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
             sa += c <= F ? va[c] : 0.f;
             if constexpr (!ConstW) sb += c <= F ? vb[c] : 0.f;
         }
+        if constexpr (TileMean) {
+#pragma unroll
+            for (int c = 0; c < 14; ++c) su += c < F ? vu[c] : 0.f;
+            su += vu[14];
+        }
+        sa *= P.inv_x;
+        if constexpr (!ConstW) sb *= P.inv_w;
         wave_sync_lds();
         if (cand) {
             const int32_t s = (int32_t)(kc * q);
-            const double m = (dhi - dlo) * a.inv_d;
+            const double m = TileMean ? (double)(su * P.inv_x) * a.inv_d : (dhi - dlo) * a.inv_d;
             if (m > a.depth_min) {
                 const float rs = (float)(m * a.os2);
                 const double delta =
@@ -466,83 +556,95 @@ __device__ __forceinline__ ArgMin mfma_width(const float* __restrict__ X,
 // Short strides (q < kToepQ) on the matrix cores as Toeplitz tiles: with rows i of a tile
 // starting at sample q (256 T + 16 i) and column j holding the template shifted by q j,
 //     C(i, j) = sum_k X[q (256 T + 16 i) + k] S[k - q j] = A(candidate 256 T + 16 i + j),
-// k running over the d + 15 q taps where a column can be non-zero. No diagonal sums: the
-// result register v of lane l is candidate 256 T + 16 (4 (l / 16) + v) + l % 16. Four
-// tiles (1024 candidates) share each template operand. The template rows carry kTmplPad
-// (>= 15 q + 32) zeros on both sides, so taps past a shifted template read zeros, and the
-// sample plane is zero past the data (reads stay below N + d + 15 q + 32).
+// k running over the d + 15 q taps where a column can be non-zero (in K groups of 16). No
+// diagonal sums: the result register v of lane l is candidate 256 T + 16 (4 (l / 16) + v)
+// + l % 16. NT tiles (256 NT candidates, a step) share each template operand. The
+// template rows carry kTmplPad (>= 15 q + 16) zeros on both sides, so taps past a shifted
+// template read zeros, and the sample plane is zero past the data (reads stay below
+// N + d + 15 q + 16). T, T2: template pair planes of S and S^2 ((hi, hi) pairs; the
+// (lo, 0) pairs tlo bytes further). With a common weight the window sums of u come from
+// the same tiles with a row of d ones (formed in registers, HSet) in float32; with
+// per-sample weights from D, the float64 prefix sums of u.
 // ---------------------------------------------------------------------------------------
 constexpr int kToepQ = 24;
+constexpr int kToepTiles = 4;
+constexpr int kToepStep = 256 * kToepTiles;   // candidates per step
 
 template <bool ConstW>
-__device__ __forceinline__ ArgMin toep_width(const float* __restrict__ X,
-                                             const float* __restrict__ X2,
-                                             const float* __restrict__ T,
-                                             const float* __restrict__ T2,
+__device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __restrict__ T,
+                                             const char* __restrict__ T2, int32_t tlo,
                                              const double* __restrict__ D, WidthArgs a) {
-    constexpr int M = 4;
+    constexpr int NT = kToepTiles;
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
     const int32_t d = a.d, q = a.q;
     const int64_t K = a.K;
-    const int32_t ng = k_groups(d + 15 * q, M);         // d + 15 q taps
-    const int64_t nsuper = (K + 1023) / 1024;
+    const int32_t ng = k_groups(d + 15 * q);         // d + 15 q taps
+    const int64_t nsuper = (K + kToepStep - 1) / kToepStep;
     ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
     // sweep of step st over one plane: row `col` of tile t covers candidates
-    // 1024 st + 256 t + 16 col .. + 15 (rows wholly past the last candidate read sample 0,
-    // results unused); B operand: template value at tap 4 M kk + s shifted by q col
-    auto sweep = [&](const float* P, const float* Tp, int64_t st) {
-        GSweep<M> w;
-        w.X = P;
+    // kToepStep st + 256 t + 16 col .. + 15 (rows wholly past the last candidate read
+    // sample 0, results unused); B operand: template pairs at taps 4 kk .. + 3 shifted by
+    // q col
+    auto sweep = [&](const char* Xp, const char* Tp, int64_t st) {
+        HSweep<NT> w;
+        w.X = Xp;
         w.C = Tp;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int64_t m0 = 1024 * st + 256 * t + 16 * col;
-            w.xo[t] = (uint32_t)(4 * ((m0 < K ? m0 * q : 0) + M * kk));
+        for (int t = 0; t < NT; ++t) {
+            const int64_t m0 = kToepStep * st + 256 * t + 16 * col;
+            w.xo[t] = (uint32_t)(4 * ((m0 < K ? m0 * q : 0) + 4 * kk));
         }
-        w.co = 4 * (M * kk - q * col);
+        w.co = 4 * (4 * kk - q * col);
         return w;
     };
-    mfma_v4 acc[4], acc2[4];
-    GSet<M> s0, s1;
-    g_load(sweep(X, T, 0), 0, s0);
+    mfma_v4 acc[NT], acc2[NT];
+    HSet<NT, ConstW> s0, s1;
+    h_load(sweep(P.X, T, 0), 0, tlo, s0, d);
     for (int64_t st = 0; st < nsuper; ++st) {
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
         // the window-mean prefix sums D[s + d - 1] and D[s - 1] (0 at s = 0) of the lane's
-        // 16 candidates (1024 st + 256 t + 16 (4 kk + v) + col), requested before the
-        // sweeps so that they arrive while the MFMAs run
-        double dlo[16], dhi[16];
+        // 4 NT candidates (kToepStep st + 256 t + 16 (4 kk + v) + col), requested before
+        // the sweeps so that they arrive while the MFMAs run
+        double dlo[ConstW ? 1 : 4 * NT], dhi[ConstW ? 1 : 4 * NT];
+        if constexpr (!ConstW) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int64_t m = 1024 * st + 256 * (e >> 2) + 16 * (4 * kk + (e & 3)) + col;
-            const int64_t s = (m < K ? m : 0) * q;
-            dhi[e] = D[s + d - 1];
-            dlo[e] = D[s > 0 ? s - 1 : 0];
+            for (int e = 0; e < 4 * NT; ++e) {
+                const int64_t m = kToepStep * st + 256 * (e >> 2) + 16 * (4 * kk + (e & 3)) + col;
+                const int64_t s = (m < K ? m : 0) * q;
+                dhi[e] = D[s + d - 1];
+                dlo[e] = D[s > 0 ? s - 1 : 0];
+            }
         }
-        // A sums (and B sums with the weights); the first group of the next sweep is
-        // loaded during this one
+        // A sums (and the sums of u with a common weight, the B sums with the weights); the
+        // first group of the next sweep is loaded during this one
         if constexpr (ConstW) {
-            run_sweep<M>(acc, sweep(X, T, st), sweep(X, T, stn), ng, s0, s1);
+            run_sweep<NT, true>(acc, acc2, sweep(P.X, T, st), sweep(P.X, T, stn), ng, tlo, s0, s1,
+                                d);
         } else {
-            run_sweep<M>(acc, sweep(X, T, st), sweep(X2, T2, st), ng, s0, s1);
-            run_sweep<M>(acc2, sweep(X2, T2, st), sweep(X, T, stn), ng, s0, s1);
+            run_sweep<NT, false>(acc, acc2, sweep(P.X, T, st), sweep(P.W, T2, st), ng, tlo, s0, s1);
+            mfma_v4 unused[NT];
+            run_sweep<NT, false>(acc2, unused, sweep(P.W, T2, st), sweep(P.X, T, stn), ng, tlo, s0,
+                                 s1);
         }
-        // the statistic of the lane's 16 candidates
+        // the statistic of the lane's 4 NT candidates
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < NT; ++t) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int e = 4 * t + v;
-                const int64_t m = 1024 * st + 256 * t + 16 * (4 * kk + v) + col;
+                const int64_t m = kToepStep * st + 256 * t + 16 * (4 * kk + v) + col;
                 if (m < K) {
                     const int32_t s = (int32_t)(m * q);
-                    const double mm = (dhi[e] - (s > 0 ? dlo[e] : 0.0)) * a.inv_d;
+                    double mm;
+                    if constexpr (ConstW) mm = (double)(acc2[t][v] * P.inv_x) * a.inv_d;
+                    else mm = (dhi[e] - (s > 0 ? dlo[e] : 0.0)) * a.inv_d;
                     if (mm > a.depth_min) {
                         const float rs = (float)(mm * a.os2);
-                        const float sa = acc[t][v];
+                        const float sa = acc[t][v] * P.inv_x;
                         const double delta =
                             ConstW ? a.wc * ((double)(rs * rs) * a.ssk - 2.0 * (double)rs * (double)sa)
-                                   : (double)(rs * rs * acc2[t][v] - 2.f * rs * sa);
+                                   : (double)(rs * rs * (acc2[t][v] * P.inv_w) - 2.f * rs * sa);
                         consider(best, a.chi2f + delta, a.k, s);
                     } else {
                         consider(best, a.sentinel, a.k, s);
@@ -552,18 +654,6 @@ __device__ __forceinline__ ArgMin toep_width(const float* __restrict__ X,
         }
     }
     return best;
-}
-
-// Dispatches on the weight mode and the K-group size of stride q (mfma_m).
-template <bool ConstW>
-__device__ __forceinline__ ArgMin window_sums_mfma(const float* X, const float* X2,
-                                                   const float* C, const float* C2,
-                                                   const float* Z, const double* D,
-                                                   const WidthArgs& a) {
-    const int32_t m = mfma_m(a.q);
-    if (m == 8) return mfma_width<ConstW, 8>(X, X2, C, C2, Z, D, a);
-    if (m == 4) return mfma_width<ConstW, 4>(X, X2, C, C2, Z, D, a);
-    return mfma_width<ConstW, 2>(X, X2, C, C2, Z, D, a);
 }
 
 }  // namespace gtls
