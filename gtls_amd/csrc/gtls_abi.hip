@@ -22,7 +22,7 @@ struct gtls_ctx {
     int win_fp64 = 0, win_fp32 = 0;     // resident workgroups of windows_kernel
     int slots_sort = 0;
     int use_mfma = 1;
-    int toep_q = gtls::kToepQ;
+    int toep_q = gtls::kToepQDefault;
     int64_t chunk_bytes = 0;            // budget of one chunk's gathered samples
     std::string err;
     // kernel timing (gtls_set_timing): per period chunk, events before the prep kernel,

@@ -15,11 +15,11 @@ from . import _ffi
 from . import constants as C
 from .search import _ctx, _device, kernel_ms, set_kernel_timing
 
-TOEP_Q = (12, 16, 24)
+TOEP_Q = (8, 12, 16, 24)
 CHUNK_MB = (8 << 10, 64 << 10)
 
 
-def set_tuning(device=0, toep_q=24, chunk_mb=0):
+def set_tuning(device=0, toep_q=8, chunk_mb=0):
     """Apply a setting to the context of `device`'s current torch stream."""
     dev = _device(device)
     ctx = _ctx(dev)

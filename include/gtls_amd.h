@@ -156,12 +156,15 @@ int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off
  * records). gtls_kernel_ms waits for the last recorded event, writes the summed
  * milliseconds of the prep, windows and reduce kernels since the previous call into
  * ms[0..2], clears the records and returns the number of chunks (or a negative error).
+ * No events are recorded while the stream is being captured into a graph; more than 16384
+ * period chunks between two gtls_kernel_ms calls discard the records, and that call
+ * returns GTLS_E_SHAPE.
  */
 int gtls_set_timing(gtls_ctx* ctx, int32_t on);
 
 /*
  * Tuning of the float32 search (no reference counterpart; set by gtls_amd.tune): strides q
- * below toep_q (0 .. 24) take the Toeplitz matrix-core tiles, larger ones the
+ * below toep_q (0 .. 24, default 8) take the Toeplitz matrix-core tiles, larger ones the
  * diagonal-sum tiles; chunk_mb (0 keeps the current value) bounds the per-period buffers
  * of one period chunk. Applies to later calls on the context.
  */
