@@ -25,7 +25,8 @@ EXPORTED = (
     "gtls_create", "gtls_destroy", "gtls_set_stream", "gtls_last_error", "gtls_search_slots",
     "gtls_search_workspace_bytes", "gtls_search_batched", "gtls_search_workspace_bytes_1",
     "gtls_search", "gtls_fold_sort_workspace_bytes", "gtls_fold_sort",
-    "gtls_spectra_workspace_bytes", "gtls_spectra_batched", "gtls_transit_fwd",
+    "gtls_spectra_workspace_bytes", "gtls_spectra_batched", "gtls_spectra_batched_noalloc",
+    "gtls_transit_fwd",
     "gtls_transit_bwd", "gtls_transit_chi2", "gtls_set_timing", "gtls_kernel_ms", "gtls_set_tuning",
 )
 
@@ -74,6 +75,7 @@ def _declare(lib):
         "gtls_fold_sort": (ct.c_int, [P, P, I64, P, I64, I32, P, I64, P]),
         "gtls_spectra_workspace_bytes": (I64, [P, I64]),
         "gtls_spectra_batched": (ct.c_int, [P, I32, P, P, I64, I32, P, I64, P, P, P, P, P]),
+        "gtls_spectra_batched_noalloc": (ct.c_int, [P, I32, P, P, I64, I32, P, P, P, P, P]),
         "gtls_transit_fwd": (ct.c_int, [P, I32, I64, P, P, P, P, P, I64, P]),
         "gtls_transit_bwd": (ct.c_int, [P, I32, I64, P, P, P, P, P, P, P]),
         "gtls_transit_chi2": (ct.c_int, [P, I32, I64, P, P, P, P, P, P, P, P]),

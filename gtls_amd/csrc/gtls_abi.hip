@@ -31,6 +31,7 @@ struct gtls_ctx {
     std::vector<hipEvent_t> pool;       // events owned by the context
     size_t used = 0;                    // events recorded since the last gtls_kernel_ms
     bool overflow = false;              // more records than kMaxTimingEvents were asked for
+    double* spectra_scratch = nullptr;  // this device's copy of `scratch` (device address)
     ~gtls_ctx() {
         for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     }
@@ -341,6 +342,50 @@ int occupancy_slots(gtls_ctx* ctx, const void* kernel) {
     return per_cu * ctx->n_cu;
 }
 
+// Running-median buffer of gtls_spectra_batched_noalloc: one copy per GPU, held until
+// the process ends; its host-side name is not a device address (gtls_create looks up the
+// device address of each GPU's copy).
+constexpr int64_t kNoAllocPeriods = 1LL << 22;   // 32 MiB of float64
+static __device__ __align__(256) double scratch[kNoAllocPeriods];
+
+// The spectra kernels of both spectra entry points; med: total_periods doubles.
+int launch_spectra(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off, const double* chi2,
+                   int64_t total_periods, int32_t kernel, double* med, double* power_raw,
+                   double* power, double* sde_raw, double* sde, int32_t* status) {
+    SpecParams p;
+    p.n_curves = n_curves;
+    p.per_off = per_off;
+    p.chi2 = chi2;
+    p.kernel = kernel;
+    p.power_raw = power_raw;
+    p.power = power;
+    p.sde_raw = sde_raw;
+    p.sde = sde;
+    p.status = status;
+    hipLaunchKernelGGL(spectra_pass1_kernel, dim3(n_curves), dim3(kSpecBlock), 0, ctx->stream, p);
+    if (total_periods > 0)
+        hipLaunchKernelGGL(running_median_kernel,
+                           dim3((unsigned)((total_periods + kSpecBlock - 1) / kSpecBlock)),
+                           dim3(kSpecBlock), 0, ctx->stream, p, total_periods, med);
+    hipLaunchKernelGGL(spectra_pass2_kernel, dim3(n_curves), dim3(kSpecBlock), 0, ctx->stream, p,
+                       (const double*)med);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "spectra launch");
+    return GTLS_OK;
+}
+
+int spectra_args_ok(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off, const double* chi2,
+                    int64_t total_periods, int32_t kernel, const double* power_raw,
+                    const double* power, const double* sde_raw, const double* sde,
+                    const int32_t* status) {
+    if (n_curves <= 0 || total_periods < 0 || !per_off || !chi2 || !power_raw || !power ||
+        !sde_raw || !sde || !status)
+        return fail(ctx, GTLS_E_SHAPE, "bad spectra arguments");
+    if (kernel < 1 || kernel > kMaxMedianKernel || kernel % 2 == 0)
+        return fail(ctx, GTLS_E_SHAPE, "median kernel must be odd and <= 255");
+    return GTLS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -361,6 +406,12 @@ int gtls_create(int device, void* stream, gtls_ctx** out) {
     ctx->device = device;
     ctx->stream = (hipStream_t)stream;
     ctx->n_cu = prop.multiProcessorCount;
+    // this device's copy of the no-allocation spectra buffer (the device is current here)
+    e = hipGetSymbolAddress((void**)&ctx->spectra_scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        delete ctx;
+        return -(int)e;
+    }
     // Each period chunk ends with the tails of its kernels, so chunks are made large: up
     // to 64 GiB of per-period buffers, at most a quarter of the device memory.
     ctx->chunk_bytes = std::min<int64_t>(64LL << 30, (int64_t)(prop.totalGlobalMem / 4));
@@ -583,35 +634,35 @@ int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off
                          void* workspace, int64_t workspace_bytes, double* power_raw,
                          double* power, double* sde_raw, double* sde, int32_t* status) {
     if (!ctx) return GTLS_E_SHAPE;
-    if (n_curves <= 0 || total_periods < 0 || !per_off || !chi2 || !workspace || !power_raw ||
-        !power || !sde_raw || !sde || !status)
-        return fail(ctx, GTLS_E_SHAPE, "bad spectra arguments");
-    if (kernel < 1 || kernel > kMaxMedianKernel || kernel % 2 == 0)
-        return fail(ctx, GTLS_E_SHAPE, "median kernel must be odd and <= 255");
+    const int ok = spectra_args_ok(ctx, n_curves, per_off, chi2, total_periods, kernel,
+                                   power_raw, power, sde_raw, sde, status);
+    if (ok != GTLS_OK) return ok;
+    if (!workspace) return fail(ctx, GTLS_E_SHAPE, "NULL workspace");
     if (workspace_bytes < gtls_spectra_workspace_bytes(ctx, total_periods))
         return fail(ctx, GTLS_E_WORKSPACE, "workspace too small");
     DeviceGuard guard(ctx->device);
-    SpecParams p;
-    p.n_curves = n_curves;
-    p.per_off = per_off;
-    p.chi2 = chi2;
-    p.kernel = kernel;
-    p.power_raw = power_raw;
-    p.power = power;
-    p.sde_raw = sde_raw;
-    p.sde = sde;
-    p.status = status;
-    double* med = (double*)workspace;
-    hipLaunchKernelGGL(spectra_pass1_kernel, dim3(n_curves), dim3(kSpecBlock), 0, ctx->stream, p);
-    if (total_periods > 0)
-        hipLaunchKernelGGL(running_median_kernel,
-                           dim3((unsigned)((total_periods + kSpecBlock - 1) / kSpecBlock)),
-                           dim3(kSpecBlock), 0, ctx->stream, p, total_periods, med);
-    hipLaunchKernelGGL(spectra_pass2_kernel, dim3(n_curves), dim3(kSpecBlock), 0, ctx->stream, p,
-                       (const double*)med);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(ctx, e, "spectra launch");
-    return GTLS_OK;
+    return launch_spectra(ctx, n_curves, per_off, chi2, total_periods, kernel,
+                          (double*)workspace, power_raw, power, sde_raw, sde, status);
+}
+
+int gtls_spectra_batched_noalloc(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off,
+                                 const double* chi2, int64_t total_periods, int32_t kernel,
+                                 double* power_raw, double* power, double* sde_raw,
+                                 double* sde, int32_t* status) {
+    if (!ctx) return GTLS_E_SHAPE;
+    const int ok = spectra_args_ok(ctx, n_curves, per_off, chi2, total_periods, kernel,
+                                   power_raw, power, sde_raw, sde, status);
+    if (ok != GTLS_OK) return ok;
+    if (total_periods > kNoAllocPeriods)
+        return fail(ctx, GTLS_E_WORKSPACE,
+                    "gtls_spectra_batched_noalloc takes at most 4194304 periods per call, got " +
+                        std::to_string(total_periods) + " (use gtls_spectra_batched)");
+    if (!ctx->spectra_scratch) return fail(ctx, GTLS_E_DEVICE, "no scratch address for the device");
+    DeviceGuard guard(ctx->device);
+    // every running-median entry the second pass reads is written by this call first, so
+    // the buffer needs no clearing
+    return launch_spectra(ctx, n_curves, per_off, chi2, total_periods, kernel,
+                          ctx->spectra_scratch, power_raw, power, sde_raw, sde, status);
 }
 
 int gtls_transit_fwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,

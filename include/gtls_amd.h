@@ -149,6 +149,20 @@ int gtls_spectra_batched(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off
                          double* power, double* sde_raw, double* sde, int32_t* status);
 
 /*
+ * gtls_spectra_batched without a workspace, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * No counterpart in the reference. The running-median buffer is a file-scope device array
+ * of the library, 4194304 float64 (32 MiB) per GPU, held until the process ends; calls
+ * with more periods return GTLS_E_WORKSPACE. The call allocates nothing. Calls that
+ * overlap on one GPU (concurrent streams or threads) share the buffer and race; calls in
+ * one stream order do not.
+ */
+int gtls_spectra_batched_noalloc(gtls_ctx* ctx, int32_t n_curves, const int64_t* per_off,
+                                 const double* chi2, int64_t total_periods, int32_t kernel,
+                                 double* power_raw, double* power, double* sde_raw,
+                                 double* sde, int32_t* status);
+
+/*
  * Kernel timing of the search (no reference counterpart; for measuring the chain's kernels
  * from the calling program). With gtls_set_timing(ctx, 1) every later search call records
  * HIP events on the context's stream around each period chunk's prep, windows and reduce
