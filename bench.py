@@ -133,6 +133,9 @@ def refine_leg(args, plans, batch0, skip):
     # the kernels against the HBM roofline, bytes per model point and call: the fused loss
     # kernel reads time, flux and weight (24 B); the flux kernels 8 B of time and 4 B of
     # flux (forward: written; backward: the upstream gradient, read)
+    # (the fused kernel's models of one curve read the same points: unique bytes count each
+    # curve's points once per call)
+    unique_points = sum(len(c[0]) for c in curves)
     for k, name, per in (("chi2", "transit_chi2_kernel", 24.0),
                          ("fwd", "transit_fwd_kernel", 12.0), ("bwd", "transit_bwd_kernel", 12.0)):
         ms, calls = kt.get(k, (0.0, 0))
@@ -141,6 +144,11 @@ def refine_leg(args, plans, batch0, skip):
             out[name] = {"ms_per_call": round(ms / calls, 4), "calls": calls,
                          "algorithmic_GBps": round(gbs, 1),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+            if k == "chi2":
+                ugbs = per * unique_points * calls / (ms * 1e-3) / 1e9
+                out[name].update({"unique_points": unique_points,
+                                  "unique_GBps": round(ugbs, 1),
+                                  "unique_hbm_frac": round(ugbs / HBM_PEAK_GBS, 4)})
     log("refine: %d models, %d steps in %.2f s (%.2f ms/step), median chi2 ratio %.4f" % (
         mb.n_models, args.refine_steps, dt, out["ms_per_step"], out["chi2_ratio_median"]))
     return out
@@ -163,6 +171,44 @@ def peak_check(sde_a, pw_a, bk_a, c_b, k_b, oversampling=3):
     return same_p, same_w, abs(float(sde_a) - float(sd)) / abs(float(sd))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def spectra_of(chi2, oversampling=3):
+    from gtls_amd import spectra as SP
+    _, _, pw, _, sd = SP.spectra(SP.mask_outliers(np.asarray(chi2, np.float64)), oversampling)
+    return np.asarray(pw), float(sd)
+
+
+def compare_searches(bp, a, b, oversampling=3):
+    """Per curve of a batch: searches a and b (chi2, best_k per period, packed order) give the
+    same best period (SDE peak) and the same best width there; max relative chi2 difference
+    over all periods and max relative SDE difference."""
+    ca, ka = bp.unpack(a[0]), bp.unpack(a[1])
+    cb, kb = bp.unpack(b[0]), bp.unpack(b[1])
+    same_p = same_w = 0
+    max_rel = max_sde = 0.0
+    for i in range(bp.n_curves):
+        ok = np.isfinite(cb[i])
+        if ok.any():
+            max_rel = max(max_rel, float(np.max(np.abs(ca[i][ok] - cb[i][ok]) / np.abs(cb[i][ok]))))
+        pa, sa = spectra_of(ca[i], oversampling)
+        pb, sb = spectra_of(cb[i], oversampling)
+        ia, ib = int(np.argmax(pa)), int(np.argmax(pb))
+        same_p += ia == ib
+        same_w += ia == ib and int(ka[i][ia]) == int(kb[i][ib])
+        max_sde = max(max_sde, abs(sa - sb) / abs(sb))
+    return same_p, same_w, max_rel, max_sde
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,9 +219,16 @@ def main():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work")
+    ap.add_argument("--weighted-curves", type=int, default=64,
+                    help="curves of the per-sample-dy leg (0: skip)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-full-check", action="store_true",
+                    help="skip the float64 search of a whole batch (parity of every curve)")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager search calls instead of graph replays")
     ap.add_argument("--refine-curves", type=int, default=256)
     ap.add_argument("--refine-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -195,10 +248,12 @@ def main():
     from gtls_amd import batch as B
     from gtls_amd import search as S
     from gtls_amd import synthetic
+    from gtls_amd import tune
     from gtls_amd.grid import skip_factor
+    from gtls_amd.validate import validate_inputs
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.copy_calibration:
+        assert torch.cuda.is_available(), "bench.py needs a GPU"
         # a plain copy of 2^28 float32 (1 GiB read, 1 GiB written), three launches
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -210,10 +265,18 @@ def main():
         return
     skip = skip_factor(0.125)
 
+    # host planning (grids, templates, width windows) before the GPU runtime starts
     t0 = time.time()
     curves = synthetic.batch(args.curves, seed=args.seed)
+    tp = time.perf_counter()
     plans = B.build_plans(curves, {})
+    t_plan = time.perf_counter() - tp
     del curves
+    workers = min(16, os.cpu_count() or 1)
+    log("planned %d curves in %.1f s on %d processes (%.2f ms per curve)" % (
+        len(plans), t_plan, workers, t_plan * 1e3 / len(plans)))
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
     n_batches = args.curves // args.batch
     batches = []
     for b in range(n_batches):
@@ -225,22 +288,52 @@ def main():
         len(plans), n_batches, sum(p.n for p in plans), sum(len(p.periods) for p in plans),
         macs.sum(), time.time() - t0))
 
+    tuning = None
+    if not args.no_tune:
+        ts = time.perf_counter()
+        choice, times = tune.autotune(batches[0][1], skip, n_sample=32, repeats=1)
+        tuning = {"choice": {"toep_q": choice[0], "chunk_mb": choice[1],
+                             "wg_per_cu": choice[2]},
+                  "candidates_ms": {"%d/%d/%d" % k: round(v, 2) for k, v in times.items()},
+                  "seconds": round(time.perf_counter() - ts, 1),
+                  "sample": "32 curves of batch 0 spread over its sizes, one timed call each"}
+        log("autotune: toep_q %d, chunk %d MiB, %d workgroups/CU (%.1f s)" % (
+            choice + (tuning["seconds"],)))
+
+    # one graph per batch, all over one workspace (replayed in stream order), each with
+    # the library's per-chunk kernel events captured
+    graphs = []
+    if not args.eager:
+        need = max(db.workspace_bytes("fp32c", skip) for _, db in batches)
+        ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+        for bp, db in batches:
+            graphs.append(db.capture("fp32c", skip, ws=ws, timing=True))
+        log("captured %d graphs over a %.1f GiB workspace" % (len(graphs), need / 2 ** 30))
+
     def step(i, ev=None):
         bp, db = batches[i % n_batches]
         if ev is not None:
             ev[0].record()
-        chi2, _, _ = db.search("fp32c", skip)
+        if graphs:
+            graphs[i % n_batches].replay()
+            chi2 = db.chi2
+        else:
+            chi2, _, _ = db.search("fp32c", skip)
         if ev is not None:
             ev[1].record()
         db.spectra(chi2, 3, sync=False)
 
-    # per-kernel HIP events around every period chunk's kernels (the library keeps and
-    # reuses them; the warmup creates them)
-    S.set_kernel_timing(0, True)
+    def step_kernel_ms(i):
+        if graphs:
+            return B.graph_kernel_ms(graphs[i % n_batches])
+        return S.kernel_ms(0)
+
+    if not graphs:
+        S.set_kernel_timing(0, True)
     for i in range(args.warmup):
         step(i)
         torch.cuda.synchronize()
-        S.kernel_ms(0)
+        step_kernel_ms(i)
         log("warmup step %d done" % i)
 
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -251,10 +344,11 @@ def main():
     for i in range(args.steps):
         step(i, evs[i])
         torch.cuda.synchronize()          # a progress line per finished step
-        kern_ms.append(S.kernel_ms(0))
+        kern_ms.append(step_kernel_ms(i))
         log("step %d done" % i)
     elapsed = time.perf_counter() - t_start
-    S.set_kernel_timing(0, False)
+    if not graphs:
+        S.set_kernel_timing(0, False)
     prep_ms = np.array([k[0][0] for k in kern_ms])
     win_ms = np.array([k[0][1] for k in kern_ms])
     red_ms = np.array([k[0][2] for k in kern_ms])
@@ -263,8 +357,8 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     n_done = args.batch * args.steps
     value = n_done / elapsed * 3600.0
-    log("timed %d steps: %.1f ms/step, search launches %.1f ms on average" % (
-        args.steps, ms_per_step, search_ms.mean()))
+    log("timed %d steps: %.1f ms/step, search %.1f ms on average (prep %.1f, windows %.1f)" % (
+        args.steps, ms_per_step, search_ms.mean(), prep_ms.mean(), win_ms.mean()))
 
     # per step: the useful window FLOPs (2 per MAC) and algorithmic bytes of its batch
     step_curves = [np.arange((i % n_batches) * args.batch, (i % n_batches + 1) * args.batch)
@@ -283,6 +377,13 @@ def main():
                 "traffic": traffic.get("windows_kernel") if traffic else None,
                 "traffic_source": (traffic_src + " (batch 0)") if traffic_src else None,
                 "kernel": "windows_kernel: every launch of a step (one per period chunk)",
+                "peak_note": "useful float32-equivalent window FLOPs against the FP32 "
+                             "matrix peak; the tiles run on 16x16x32 f16 MFMAs over "
+                             "(hi, lo) pairs (2.5 PFLOP/s dense f16 peak)",
+                "timing": "HIP events the library records around every chunk's kernels on "
+                          "the launch stream, " + ("captured in each batch's graph and "
+                                                   "recorded by every replay" if graphs
+                                                   else "eager calls"),
                 "launch_ms": round(float(win_ms.mean()), 3),
                 "launch_ms_per_step": [round(float(x), 3) for x in win_ms],
                 "launches_per_step": [int(x) for x in chunks],
@@ -309,9 +410,77 @@ def main():
         for j in range(args.batch):
             ci = b * args.batch + j
             res[ci] = (spec["SDE"][j], spec["power"][j], bk[j])
-    extra = {"batch_curves": args.batch, "batches_in_set": n_batches}
+    extra = {"batch_curves": args.batch, "batches_in_set": n_batches,
+             "step": "graph replay" if graphs else "eager search call",
+             "host_planning_ms_per_curve": round(t_plan * 1e3 / len(plans), 3),
+             "host_planning_note": "grids, templates and width windows on %d host processes, "
+                                   "outside value (as in the plain leg)" % workers,
+             "tuning": tuning}
     done = np.array(sorted(res))
     total_macs = macs[done].sum()
+
+    if not args.no_full_check:
+        # every curve of batch 0: the batched float32 search against the batched float64 one
+        bp0, db0 = batches[0]
+        if graphs:
+            graphs[0].replay()
+        else:
+            db0.search("fp32c", skip)
+        a = (db0.chi2.cpu().numpy().copy(), db0.best_k.cpu().numpy().copy())
+        ts = time.perf_counter()
+        db0.search("fp64", skip)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - ts
+        b = (db0.chi2.cpu().numpy().copy(), db0.best_k.cpu().numpy().copy())
+        sp, sw, mrel, msde = compare_searches(bp0, a, b)
+        extra["full_batch_check"] = {
+            "curves": bp0.n_curves, "against": "batched float64 search of the same batch",
+            "same_best_period": "%d/%d" % (sp, bp0.n_curves),
+            "same_best_width": "%d/%d" % (sw, bp0.n_curves),
+            "max_rel_chi2_err": mrel, "max_rel_sde_err": msde,
+            "fp64_batched_curves_per_hour": round(bp0.n_curves / t64 * 3600.0, 1)}
+        log("full batch 0: best period %d/%d, width %d/%d, max rel chi2 %.2e, max dSDE %.2e"
+            % (sp, bp0.n_curves, sw, bp0.n_curves, mrel, msde))
+        if graphs:
+            graphs[0].replay()          # batch 0's float32 results back in place
+            torch.cuda.synchronize()
+
+    if args.weighted_curves > 0:
+        # per-sample dy (sigma U(0.5, 1.5)) on the first curves of the set: the weighted
+        # path (two sample planes, float64 window means)
+        rng = np.random.default_rng(args.seed + 100)
+        k = min(args.weighted_curves, len(plans))
+        wc = []
+        for p in plans[:k]:
+            sig = float(np.std(p.y))
+            wc.append((p.t, p.y, sig * rng.uniform(0.5, 1.5, p.n)))
+        wplans = B.build_plans(wc, {})
+        wbp = B.BatchPlan(wplans)
+        wdb = B.DeviceBatch(wbp, 0)
+        wdb.search("fp32c", skip)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        wdb.search("fp32c", skip)
+        e1.record()
+        torch.cuda.synchronize()
+        tw = e0.elapsed_time(e1) * 1e-3
+        a = (wdb.chi2.cpu().numpy().copy(), wdb.best_k.cpu().numpy().copy())
+        wdb.search("fp64", skip)
+        torch.cuda.synchronize()
+        b = (wdb.chi2.cpu().numpy().copy(), wdb.best_k.cpu().numpy().copy())
+        sp, sw, mrel, msde = compare_searches(wbp, a, b)
+        wm = float(np.sum([window_macs(p, skip) for p in wplans]))
+        extra["weighted"] = {
+            "curves": k, "dy": "std(flux) x U(0.5, 1.5) per sample",
+            "search_seconds": round(tw, 3),
+            "curves_per_hour": round(k / tw * 3600.0, 1),
+            "window_tflops": round(2 * wm / tw / 1e12, 3),
+            "same_best_period_vs_fp64": "%d/%d" % (sp, k),
+            "same_best_width_vs_fp64": "%d/%d" % (sw, k),
+            "max_rel_chi2_err_vs_fp64": mrel, "max_rel_sde_err_vs_fp64": msde}
+        log("weighted: %d curves in %.2f s, best period %d/%d vs fp64, max rel chi2 %.2e" % (
+            k, tw, sp, k, mrel))
 
     if not args.no_plain:
         sample = [int(done[i]) for i in stratified(macs[done], args.plain_sample)]
@@ -374,7 +543,7 @@ def main():
             max_dsde = max(max_dsde, d)
         cpu_value = len(done) / (t_cpu * total_macs / w_cpu) * 3600.0
         cpu = {"value": round(cpu_value, 2), "unit": "light curves/hour", "cores": cores,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(),
                "sample": "%d curves of the set spread over its work distribution "
                          "(N %d-%d; %.1f s of FP64 oracle search on %d threads), "
                          "extrapolated by window MACs" % (
@@ -398,7 +567,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "fp32+fp64 (f16 hi/lo pairs on the matrix cores)",
         "data": "synthetic (seeded light curves with injected limb-darkened transits)",
         "config": {"workload": "batched TLS search, %d-curve set in batches of %d, "
                                "N log-U(2000, 70000), cadence 2/30 min, own period grid "

@@ -28,6 +28,7 @@ EXPORTED = (
     "gtls_spectra_workspace_bytes", "gtls_spectra_batched", "gtls_spectra_batched_noalloc",
     "gtls_transit_fwd",
     "gtls_transit_bwd", "gtls_transit_chi2", "gtls_set_timing", "gtls_kernel_ms", "gtls_set_tuning",
+    "gtls_set_graph_timing",
 )
 
 P = ct.c_void_p
@@ -81,7 +82,8 @@ def _declare(lib):
         "gtls_transit_chi2": (ct.c_int, [P, I32, I64, P, P, P, P, P, P, P, P]),
         "gtls_set_timing": (ct.c_int, [P, I32]),
         "gtls_kernel_ms": (I64, [P, ct.POINTER(F64)]),
-        "gtls_set_tuning": (ct.c_int, [P, I32, I64]),
+        "gtls_set_tuning": (ct.c_int, [P, I32, I64, I32]),
+        "gtls_set_graph_timing": (ct.c_int, [P, P, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -8,6 +8,7 @@ per-period outputs are mapped back to the caller's curve order.
 from __future__ import annotations
 
 import ctypes as ct
+import multiprocessing
 import os
 from concurrent.futures import ProcessPoolExecutor
 
@@ -26,13 +27,16 @@ def _plan_one(item):
 
 
 def build_plans(curves, kwargs=None, workers=None):
-    """CurvePlans of (t, y, dy) tuples, built in a process pool."""
+    """CurvePlans of (t, y, dy) tuples, built in a process pool. The workers are started
+    fresh (spawn), not forked: a fork of a process whose GPU runtime is initialised copies
+    its threads' locks in whatever state they hold, and a worker can then block forever."""
     kwargs = kwargs or {}
     items = [(c[0], c[1], c[2], kwargs) for c in curves]
     workers = workers or min(16, os.cpu_count() or 1)
     if workers <= 1 or len(items) < 8:
         return [_plan_one(it) for it in items]
-    with ProcessPoolExecutor(max_workers=workers) as ex:
+    with ProcessPoolExecutor(max_workers=workers,
+                             mp_context=multiprocessing.get_context("spawn")) as ex:
         return list(ex.map(_plan_one, items, chunksize=max(1, len(items) // (4 * workers))))
 
 
@@ -108,6 +112,8 @@ def size_groups(bp, ratio=2.0):
 class DeviceBatch:
     """Device-resident BatchPlan; search() enqueues one persistent search launch per size
     group of curves (size_groups)."""
+
+    MAX_TIMED_CHUNKS = 4096       # period chunks a timed graph can hold stamps for
 
     def __init__(self, bp: BatchPlan, device=0):
         import torch
@@ -226,34 +232,46 @@ class DeviceBatch:
         return dict(power=power, power_raw=power_raw, SDE=sde, SDE_raw=sde_raw,
                     device=status == 0)
 
-    def capture(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
+    def capture(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, ws=None,
+                timing=False):
         """A HIP graph (torch.cuda.CUDAGraph) of search(): replay() re-runs every size
-        group's search call into chi2 / best_k / best_s with no host work. The graph gets a
-        workspace of its own, allocated before the capture and kept alive with the graph
-        (attribute `workspace` of the returned graph, and this batch's `graph_workspaces`),
-        so no later search or capture can free or share the memory its kernels write. The
-        search runs once on the graph's stream first, so that the library context exists
-        before the capture; every kernel of the call is captured (no memset nodes)."""
+        group's search call into chi2 / best_k / best_s with no host work.
+
+        The graph gets a workspace of its own, allocated before the capture and kept alive
+        with the graph (attribute `workspace`), so no later search or capture can free or
+        share the memory its kernels write; `ws` passes one instead (the caller then keeps
+        it alive and must not replay graphs that share it concurrently). The capture runs
+        with a library context of its own (attribute `gtls_ctx`); with `timing`, its
+        per-chunk kernel events are captured too and graph_kernel_ms(g) reads the figures
+        of the last replay (wall-clock stamps written by one-thread kernels between the
+        chunks' kernels, gtls_set_graph_timing). Every kernel of the call is captured (no
+        memset nodes)."""
         import torch
-        ws = torch.empty(self.workspace_bytes(mode, skip_point, depth_min), dtype=torch.uint8,
-                         device=self.dev)
+        if ws is None:
+            ws = torch.empty(self.workspace_bytes(mode, skip_point, depth_min),
+                             dtype=torch.uint8, device=self.dev)
         s = torch.cuda.Stream(self.dev)
         s.wait_stream(torch.cuda.current_stream(self.dev))
-        with torch.cuda.stream(s):
-            self.search(mode, skip_point, depth_min, ws=ws)
-        torch.cuda.current_stream(self.dev).wait_stream(s)
+        ctx = _ffi.Context(self.dev.index, s.cuda_stream)
+        stamps = None
+        if timing:
+            stamps = torch.zeros(4 * self.MAX_TIMED_CHUNKS, dtype=torch.int64, device=self.dev)
+            _ffi.check(ctx.handle, _ffi.load().gtls_set_graph_timing(
+                ctx.handle, _ptr(stamps), self.MAX_TIMED_CHUNKS))
         torch.cuda.synchronize(self.dev)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=s):
-            self.search(mode, skip_point, depth_min, ws=ws)
+            self.search(mode, skip_point, depth_min, ws=ws, ctx=ctx)
         g.workspace = ws
-        self.graph_workspaces = getattr(self, "graph_workspaces", []) + [ws]
+        g.gtls_ctx = ctx
+        g.stamps = stamps
         return g
 
-    def workspace_bytes(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN):
+    def workspace_bytes(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN,
+                        ctx=None):
         """Workspace bytes of search(): the largest need over the size groups."""
         m = MODES[mode]
-        ctx = _ctx(self.dev)
+        ctx = ctx or _ctx(self.dev)
         lib = _ffi.load()
         need = 0
         for g in self.groups:
@@ -264,13 +282,15 @@ class DeviceBatch:
             need = max(need, int(nb))
         return need
 
-    def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, ws=None):
+    def search(self, mode="fp32c", skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, ws=None,
+               ctx=None):
         """Enqueue the batched search on torch's current stream (no synchronisation): one
         gtls_search_batched call per size group, each writing its slice of the outputs.
         `ws` (a uint8 device tensor of at least workspace_bytes()) is the workspace; by
-        default the one shared by searches on the current stream."""
+        default the one shared by searches on the current stream. `ctx`: the library
+        context (default: the one of the current stream)."""
         m = MODES[mode]
-        ctx = _ctx(self.dev)
+        ctx = ctx or _ctx(self.dev)
         lib = _ffi.load()
         structs = [self.batch_struct(skip_point, depth_min, g) for g in self.groups]
         need = 0
@@ -293,3 +313,13 @@ class DeviceBatch:
                 ct.c_void_p(self.best_s.data_ptr() + 4 * g["p0"]))
             _ffi.check(ctx.handle, code)
         return self.chi2, self.best_k, self.best_s
+
+
+def graph_kernel_ms(g):
+    """(prep, windows, reduce) milliseconds of the last replay of a graph captured with
+    DeviceBatch.capture(timing=True), and its number of period chunks."""
+    ms = (ct.c_double * 3)()
+    n = _ffi.load().gtls_kernel_ms(g.gtls_ctx.handle, ms)
+    if n < 0:
+        _ffi.check(g.gtls_ctx.handle, int(n))
+    return (ms[0], ms[1], ms[2]), int(n)

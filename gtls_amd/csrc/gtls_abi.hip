@@ -23,6 +23,7 @@ struct gtls_ctx {
     int slots_sort = 0;
     int use_mfma = 1;
     int toep_q = gtls::kToepQDefault;
+    int wg_per_cu = 0;                  // cap on resident search workgroups per CU (0: none)
     int64_t chunk_bytes = 0;            // budget of one chunk's gathered samples
     std::string err;
     // kernel timing (gtls_set_timing): per period chunk, events before the prep kernel,
@@ -32,6 +33,10 @@ struct gtls_ctx {
     size_t used = 0;                    // events recorded since the last gtls_kernel_ms
     bool overflow = false;              // more records than kMaxTimingEvents were asked for
     double* spectra_scratch = nullptr;  // this device's copy of `scratch` (device address)
+    // graph timing (gtls_set_graph_timing): caller-owned device buffer of wall-clock stamps
+    uint64_t* stamps = nullptr;
+    int64_t stamp_cap = 0;              // stamps the buffer holds
+    double wall_khz = 0.0;              // wall-clock counter rate
     ~gtls_ctx() {
         for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     }
@@ -43,6 +48,9 @@ using namespace gtls;
 
 constexpr int64_t kAlign = 256;
 constexpr size_t kMaxTimingEvents = 1 << 16;   // records kept between gtls_kernel_ms calls
+
+// The device wall clock (a constant-rate counter) at the time this kernel runs.
+__global__ void wall_stamp_kernel(uint64_t* out) { *out = wall_clock64(); }
 
 int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -133,6 +141,10 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     L.slot_bytes = s;
     L.n_slots = (mode == GTLS_MODE_FP64) ? ctx->prep_fp64 : ctx->prep_fp32;
     L.n_win = (mode == GTLS_MODE_FP64) ? ctx->win_fp64 : ctx->win_fp32;
+    if (ctx->wg_per_cu > 0) {
+        L.n_slots = std::min<int64_t>(L.n_slots, (int64_t)ctx->n_cu * ctx->wg_per_cu);
+        L.n_win = std::min<int64_t>(L.n_win, (int64_t)ctx->n_cu * ctx->wg_per_cu);
+    }
     L.slots = off;
     off = align_up(off + L.slot_bytes * L.n_slots);
     // per period of a chunk: gathered samples N + widest width of the wrap, plus the zeros
@@ -292,18 +304,27 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.chi2 = chi2;
     p.best_k = best_k;
     p.best_s = best_s;
-    // with timing on, an event before and after each of the chunk's three kernels
-    // Events are not recorded while the stream is being captured into a graph (they would
-    // become graph nodes that are never timed), nor past kMaxTimingEvents records.
+    // With timing on, a mark before and after each of the chunk's three kernels. Mode 1:
+    // HIP events, for calls that run now (nothing while the stream is being captured into a
+    // graph). Mode 2: a one-thread kernel that stores the device wall clock into the next
+    // slot of the caller's stamp buffer, for calls run or captured (every replay of a graph
+    // writes them again). Never past kMaxTimingEvents records / the buffer.
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (ctx->timing) {
         const hipError_t ce = hipStreamIsCapturing(st, &cap);
         if (ce != hipSuccess) return hip_fail(ctx, ce, "stream capture status");
     }
-    const bool timed = ctx->timing && cap == hipStreamCaptureStatusNone;
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    const bool timed = (ctx->timing == 1 && !capturing) || ctx->timing == 2;
     if (timed && ctx->used + 4 * (size_t)L.n_chunks > kMaxTimingEvents) ctx->overflow = true;
+    if (timed && ctx->timing == 2 && (int64_t)(ctx->used + 4 * (size_t)L.n_chunks) > ctx->stamp_cap)
+        ctx->overflow = true;
     auto mark = [&]() -> hipError_t {
         if (!timed || ctx->overflow) return hipSuccess;
+        if (ctx->timing == 2) {
+            hipLaunchKernelGGL(wall_stamp_kernel, dim3(1), dim3(1), 0, st, ctx->stamps + ctx->used++);
+            return hipSuccess;
+        }
         if (ctx->used == ctx->pool.size()) {
             hipEvent_t e;
             const hipError_t r = hipEventCreate(&e);
@@ -406,6 +427,9 @@ int gtls_create(int device, void* stream, gtls_ctx** out) {
     ctx->device = device;
     ctx->stream = (hipStream_t)stream;
     ctx->n_cu = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess)
+        ctx->wall_khz = (double)khz;
     // this device's copy of the no-allocation spectra buffer (the device is current here)
     e = hipGetSymbolAddress((void**)&ctx->spectra_scratch, HIP_SYMBOL(scratch));
     if (e != hipSuccess) {
@@ -441,10 +465,12 @@ int gtls_set_stream(gtls_ctx* ctx, void* stream) {
 
 const char* gtls_last_error(const gtls_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb) {
+int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb, int32_t wg_per_cu) {
     if (!ctx) return GTLS_E_SHAPE;
     if (toep_q < 0 || toep_q > kToepQ) return fail(ctx, GTLS_E_SHAPE, "toep_q out of [0, 24]");
     if (chunk_mb < 0) return fail(ctx, GTLS_E_SHAPE, "chunk_mb < 0");
+    if (wg_per_cu < 0) return fail(ctx, GTLS_E_SHAPE, "wg_per_cu < 0");
+    ctx->wg_per_cu = wg_per_cu;
     ctx->toep_q = toep_q;
     if (chunk_mb > 0) ctx->chunk_bytes = chunk_mb << 20;
     return GTLS_OK;
@@ -470,6 +496,18 @@ int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms) {
     }
     const size_t n = ctx->used / 4 * 4;
     DeviceGuard guard(ctx->device);
+    if (ctx->timing == 2) {
+        // graph timing: the stamps of the last run or replay
+        std::vector<uint64_t> h(n);
+        hipError_t e = hipDeviceSynchronize();
+        if (e == hipSuccess && n > 0)
+            e = hipMemcpy(h.data(), ctx->stamps, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(ctx, e, "timing stamps read");
+        for (size_t i = 0; i < n; i += 4)
+            for (int k = 0; k < 3; ++k)
+                ms[k] += (double)(int64_t)(h[i + k + 1] - h[i + k]) / ctx->wall_khz;
+        return (int64_t)(n / 4);
+    }
     if (n > 0) {
         const hipError_t e = hipEventSynchronize(ctx->pool[n - 1]);
         if (e != hipSuccess) return hip_fail(ctx, e, "timing sync");
@@ -484,6 +522,25 @@ int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms) {
     }
     ctx->used = 0;
     return (int64_t)(n / 4);
+}
+
+int gtls_set_graph_timing(gtls_ctx* ctx, uint64_t* stamps, int64_t max_chunks) {
+    if (!ctx) return GTLS_E_SHAPE;
+    ctx->used = 0;
+    ctx->overflow = false;
+    if (!stamps) {
+        ctx->timing = 0;
+        ctx->stamps = nullptr;
+        ctx->stamp_cap = 0;
+        return GTLS_OK;
+    }
+    if (max_chunks <= 0 || max_chunks > (int64_t)kMaxTimingEvents / 4)
+        return fail(ctx, GTLS_E_SHAPE, "max_chunks out of [1, 16384]");
+    if (!(ctx->wall_khz > 0.0)) return fail(ctx, GTLS_E_DEVICE, "no wall-clock rate");
+    ctx->timing = 2;
+    ctx->stamps = stamps;
+    ctx->stamp_cap = 4 * max_chunks;
+    return GTLS_OK;
 }
 
 int32_t gtls_search_slots(const gtls_ctx* ctx) { return ctx ? ctx->prep_fp32 : 0; }

@@ -1,13 +1,17 @@
 """Autotuning of the float32 batched search on the device it runs on.
 
 The tunables are those of gtls_set_tuning: the stride below which a width takes the
-Toeplitz matrix-core tiles instead of the diagonal-sum tiles, and the buffer budget of one
-period chunk (which sets how many kernel tails a call pays). `autotune` times one search
+Toeplitz matrix-core tiles instead of the diagonal-sum tiles, the buffer budget of one
+period chunk (which sets how many kernel tails a call pays), and the resident workgroups
+per CU of the persistent search kernels (waves per SIMD: one or two). The K group of the
+tiles is fixed at 16 samples by the matrix instruction (16x16x32 over sample pairs). `autotune` times one search
 call of a sample batch (curves spread over the batch's sizes) under each candidate setting
 with the library's per-kernel HIP events, and applies the fastest to the context of the
 device's current stream; later searches from this process on that stream use it.
 """
 from __future__ import annotations
+
+import itertools
 
 import numpy as np
 
@@ -17,13 +21,16 @@ from .search import _ctx, _device, kernel_ms, set_kernel_timing
 
 TOEP_Q = (8, 12, 16, 24)
 CHUNK_MB = (8 << 10, 64 << 10)
+WG_PER_CU = (1, 2)
+DEFAULT = (8, 64 << 10, 0)
 
 
-def set_tuning(device=0, toep_q=8, chunk_mb=0):
+def set_tuning(device=0, toep_q=8, chunk_mb=0, wg_per_cu=0):
     """Apply a setting to the context of `device`'s current torch stream."""
     dev = _device(device)
     ctx = _ctx(dev)
-    _ffi.check(ctx.handle, _ffi.load().gtls_set_tuning(ctx.handle, int(toep_q), int(chunk_mb)))
+    _ffi.check(ctx.handle, _ffi.load().gtls_set_tuning(ctx.handle, int(toep_q), int(chunk_mb),
+                                                       int(wg_per_cu)))
 
 
 def sample_batch(db, n=32):
@@ -36,8 +43,8 @@ def sample_batch(db, n=32):
 
 
 def autotune(db, skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, n_sample=32, repeats=2):
-    """Times the sample's search under every (toep_q, chunk_mb) candidate (the best of
-    `repeats` calls each, after one untimed call) and applies the fastest. Returns
+    """Times the sample's search under every (toep_q, chunk_mb, wg_per_cu) candidate (the
+    best of `repeats` calls each, after one untimed call) and applies the fastest. Returns
     (best setting, {setting: milliseconds})."""
     import torch
     sb = sample_batch(db, n_sample)
@@ -45,19 +52,18 @@ def autotune(db, skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, n_sample=32, repea
     times = {}
     set_kernel_timing(dev, True)
     try:
-        for q in TOEP_Q:
-            for mb in CHUNK_MB:
-                set_tuning(dev, q, mb)
+        for setting in itertools.product(TOEP_Q, CHUNK_MB, WG_PER_CU):
+            set_tuning(dev, *setting)
+            sb.search("fp32c", skip_point, depth_min)
+            torch.cuda.synchronize(db.dev)
+            kernel_ms(dev)
+            best = np.inf
+            for _ in range(repeats):
                 sb.search("fp32c", skip_point, depth_min)
                 torch.cuda.synchronize(db.dev)
-                kernel_ms(dev)
-                best = np.inf
-                for _ in range(repeats):
-                    sb.search("fp32c", skip_point, depth_min)
-                    torch.cuda.synchronize(db.dev)
-                    (pm, wm, rm), _ = kernel_ms(dev)
-                    best = min(best, pm + wm + rm)
-                times[(q, mb)] = best
+                (pm, wm, rm), _ = kernel_ms(dev)
+                best = min(best, pm + wm + rm)
+            times[setting] = best
     finally:
         set_kernel_timing(dev, False)
     choice = min(times, key=times.get)

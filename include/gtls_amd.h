@@ -170,19 +170,33 @@ int gtls_spectra_batched_noalloc(gtls_ctx* ctx, int32_t n_curves, const int64_t*
  * records). gtls_kernel_ms waits for the last recorded event, writes the summed
  * milliseconds of the prep, windows and reduce kernels since the previous call into
  * ms[0..2], clears the records and returns the number of chunks (or a negative error).
- * No events are recorded while the stream is being captured into a graph; more than 16384
- * period chunks between two gtls_kernel_ms calls discard the records, and that call
- * returns GTLS_E_SHAPE.
+ * No events are recorded while the stream is being captured into a graph (see
+ * gtls_set_graph_timing); more than 16384 period chunks between two gtls_kernel_ms calls
+ * discard the records, and that call returns GTLS_E_SHAPE.
  */
 int gtls_set_timing(gtls_ctx* ctx, int32_t on);
+
+/*
+ * Kernel timing that survives graph capture (no reference counterpart). With a device
+ * buffer `stamps` of 4 * max_chunks uint64 (caller-owned, max_chunks <= 16384), every later
+ * search call on the context -- run now or captured into a graph -- launches a one-thread
+ * kernel before and after each period chunk's prep, windows and reduce kernels that
+ * stores the device wall clock into the next slot; every replay of a captured graph writes
+ * the same slots again. gtls_kernel_ms then waits for the device and returns the figures
+ * of the last run or replay (without clearing them); stamps = NULL switches this off.
+ * Use one context per timed graph.
+ */
+int gtls_set_graph_timing(gtls_ctx* ctx, uint64_t* stamps, int64_t max_chunks);
 
 /*
  * Tuning of the float32 search (no reference counterpart; set by gtls_amd.tune): strides q
  * below toep_q (0 .. 24, default 8) take the Toeplitz matrix-core tiles, larger ones the
  * diagonal-sum tiles; chunk_mb (0 keeps the current value) bounds the per-period buffers
- * of one period chunk. Applies to later calls on the context.
+ * of one period chunk; wg_per_cu (0: as many as fit) caps the resident workgroups per CU
+ * of the persistent search kernels (their waves per SIMD). Applies to later calls on the
+ * context.
  */
-int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb);
+int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb, int32_t wg_per_cu);
 int64_t gtls_kernel_ms(gtls_ctx* ctx, double* ms);
 
 /*

@@ -60,3 +60,27 @@ def test_two_graphs_keep_their_own_workspaces(gpu):
             assert torch.equal(db.best_k, want[name][1]) and torch.equal(db.best_s, want[name][2])
             bigger.search("fp32c", 8)    # grows (reallocates) the shared workspace
             torch.cuda.synchronize()
+
+
+def test_graph_kernel_timing_reads_each_replay(gpu):
+    """A graph captured with timing records the library's per-chunk events on every replay:
+    the figures are positive, their chunk count is the eager call's, and their sum stays
+    within the replay's own wall time."""
+    curves = synthetic.batch(10, seed=31, n_min=3000, n_max=9000)
+    db = B.DeviceBatch(B.BatchPlan(B.build_plans(curves, {})), 0)
+    S.set_kernel_timing(0, True)
+    db.search("fp32c", 8)
+    torch.cuda.synchronize()
+    (_, _, _), n_eager = S.kernel_ms(0)
+    S.set_kernel_timing(0, False)
+    g = db.capture("fp32c", 8, timing=True)
+    for r in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        (pm, wm, rm), n = B.graph_kernel_ms(g)
+        assert n == n_eager > 0, (n, n_eager)
+        assert pm > 0 and wm > 0 and rm >= 0, (pm, wm, rm)
+        assert pm + wm + rm <= e0.elapsed_time(e1) * 1.05 + 0.05, r

@@ -18,15 +18,15 @@ def test_autotune_picks_a_candidate_and_keeps_results(gpu):
     torch.cuda.synchronize()
     try:
         choice, times = tune.autotune(db, n_sample=12, repeats=1)
-        assert len(times) == len(tune.TOEP_Q) * len(tune.CHUNK_MB)
+        assert len(times) == len(tune.TOEP_Q) * len(tune.CHUNK_MB) * len(tune.WG_PER_CU)
         assert all(np.isfinite(v) and v > 0 for v in times.values())
         assert choice in times and times[choice] == min(times.values())
-        for q in tune.TOEP_Q:
-            tune.set_tuning(0, q, 0)
+        for q, wg in ((q, wg) for q in tune.TOEP_Q for wg in tune.WG_PER_CU):
+            tune.set_tuning(0, q, 0, wg)
             c = db.search("fp32c", 8)[0]
             torch.cuda.synchronize()
             ok = torch.isfinite(c0)
             rel = ((c - c0).abs() / c0.abs())[ok]
-            assert float(rel.max()) < 2e-6, q
+            assert float(rel.max()) < 2e-6, (q, wg)
     finally:
-        tune.set_tuning(0, 24, 64 << 10)
+        tune.set_tuning(0, *tune.DEFAULT)
