@@ -16,11 +16,17 @@ from . import constants as C
 
 
 def cleaned_array(t, y, dy=None):
-    """Drop samples where any of t, y (, dy) is NaN, <= 0 or infinite."""
-    t = np.asarray(t, dtype=np.float64).ravel()
-    y = np.asarray(y, dtype=np.float64).ravel()
-    cols = [t, y] if dy is None else [t, y, np.asarray(dy, dtype=np.float64).ravel()]
-    n = min(len(c) for c in cols)
+    """Drop samples where any of t, y (, dy) is NaN, None, <= 0 or infinite, over the
+    first len(y) samples (the reference walks range(len(y)), src/gputls/helpers.py:16-59;
+    a shorter t or dy is an error)."""
+    t = np.asarray(t, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    cols = [t, y] if dy is None else [t, y, np.asarray(dy, dtype=np.float64)]
+    if any(c.ndim != 1 for c in cols):
+        raise ValueError("Inputs (t, y, dy) must be 1-dimensional")
+    n = len(y)
+    if any(len(c) < n for c in cols):
+        raise ValueError("Arrays (t, y, dy) must be of the same dimensions")
     keep = np.ones(n, dtype=bool)
     for c in cols:
         c = c[:n]
@@ -39,10 +45,10 @@ def validate_inputs(t, y, dy):
     else:
         t, y, dy = cleaned_array(t, y, dy)
         dy = dy / np.mean(dy)
+    if np.size(t) == 0 or max(t) - min(t) <= 0:
+        raise ValueError("Time duration must positive")
     if np.size(y) < 3 or np.size(t) < 3:
         raise ValueError("Too few values in data set")
-    if max(t) - min(t) <= 0:
-        raise ValueError("Time duration must positive")
     if np.mean(y) > 1.01 or np.mean(y) < 0.99:
         warnings.warn("The mean flux should be normalized to 1, but it was found to be "
                       + str(np.mean(y)))
