@@ -57,6 +57,47 @@ def nearest_indices(targets, periods):
     return [int(np.argmin(np.abs(periods - v))) for v in targets]
 
 
+def run_power(plan, args, search):
+    """The steps of power() after the plan (src/gputls/core.py:568-999), with `search`
+    doing the chi2 search: search(period_indices, skip_point) -> (chi2, best width index,
+    best T0 index) per period, numpy. power() passes the device search; any search with
+    the same contract (the FP64 oracle, in the tests) gives the same pipeline. Returns
+    (periods, power) with args.fast, else the gtlsResult."""
+    periods = plan.periods
+    skip = skip_factor(args.T0_fit_margin)
+    raw_chi2, _, _ = search(np.arange(len(periods)), skip)
+    chi2 = mask_outliers(raw_chi2)
+    periods_m = ma.array(periods, mask=ma.getmaskarray(chi2))
+    _, _, power, _, _ = spectra(chi2, args.oversampling_factor)
+    raw_power = power.copy()
+    if args.fast:
+        return periods_m, power
+
+    cand = candidate_indices(periods, power)
+    again, _, _ = search(np.asarray(cand), C.SKIP_NONE)
+    chi2[cand] = again
+    _, _, power, _, _ = spectra(chi2, args.oversampling_factor)
+    best = cand[int(np.argmax(np.asarray(power)[cand]))]
+
+    harm = nearest_indices([periods[best] * r for r in HARMONICS], periods)
+    again, hk, hs = search(np.asarray(harm), C.SKIP_NONE)
+    chi2[harm] = again
+    _, _, power, SDE_raw, SDE = spectra(chi2, args.oversampling_factor)
+    j = int(np.argmax(np.asarray(power)[harm]))
+    best = harm[j]
+    period = float(periods[best])
+    ref = refine_period(plan, period, int(hk[j]), int(hs[j]))
+    return gtlsResult(
+        periods=periods_m, period=period, rawDuration=ref["rawDuration"],
+        durationPoints=ref["durationPoints"], rawDurations=list(plan.overview["duration"]),
+        duration=ref["duration"], depth=ref["depth"], T0=ref["T0"], SDE=SDE,
+        SDE_raw=SDE_raw, chi2=chi2, transitTimes=ref["transitTimes"], power=power,
+        snr=ref["snr"], snrPink=ref["snrPink"], snrFit=ref["snrFit"],
+        snrFitPink=ref["snrFitPink"], raw_power=raw_power, raw_chi2=raw_chi2,
+        possiblePeriodsIndices=cand, possiblePeriods=[float(periods[i]) for i in cand],
+        bestWidthIndex=int(hk[j]), bestT0Index=int(hs[j]))
+
+
 class gtls:
     """Transit least squares of limb-darkened transit models on one light curve."""
 
@@ -81,48 +122,19 @@ class gtls:
             print("Searching %d data points, %d periods from %.3f to %.3f days (%s)"
                   % (plan.n, len(periods), periods.min(), periods.max(), args.mode))
         dc = DeviceCurve(plan, self._device_id(args))
-        skip = skip_factor(args.T0_fit_margin)
         depth_min = args.transit_depth_min
 
-        def run(idx, skip_point):
-            idx = np.asarray(idx)
+        def search(idx, skip_point):
             c, k, s = dc.search(periods[idx], plan.k_lo[idx], plan.k_hi[idx], mode=args.mode,
                                 skip_point=skip_point, depth_min=depth_min)
             return c.cpu().numpy(), k.cpu().numpy(), s.cpu().numpy()
 
-        raw_chi2, _, _ = run(np.arange(len(periods)), skip)
-        chi2 = mask_outliers(raw_chi2)
-        periods_m = ma.array(periods, mask=ma.getmaskarray(chi2))
-        _, _, power, _, _ = spectra(chi2, args.oversampling_factor)
-        raw_power = power.copy()
+        out = run_power(plan, args, search)
         if args.fast:
-            return periods_m, power
-
-        cand = candidate_indices(periods, power)
-        again, _, _ = run(cand, C.SKIP_NONE)
-        chi2[cand] = again
-        _, _, power, _, _ = spectra(chi2, args.oversampling_factor)
-        best = cand[int(np.argmax(np.asarray(power)[cand]))]
-
-        harm = nearest_indices([periods[best] * r for r in HARMONICS], periods)
-        again, hk, hs = run(harm, C.SKIP_NONE)
-        chi2[harm] = again
-        _, _, power, SDE_raw, SDE = spectra(chi2, args.oversampling_factor)
-        j = int(np.argmax(np.asarray(power)[harm]))
-        best = harm[j]
-        period = float(periods[best])
-        ref = refine_period(plan, period, int(hk[j]), int(hs[j]))
-
-        self.period = period
-        self.rawDurations = list(plan.overview["duration"])
-        return gtlsResult(
-            periods=periods_m, period=period, rawDuration=ref["rawDuration"],
-            durationPoints=ref["durationPoints"], rawDurations=self.rawDurations,
-            duration=ref["duration"], depth=ref["depth"], T0=ref["T0"], SDE=SDE,
-            SDE_raw=SDE_raw, chi2=chi2, transitTimes=ref["transitTimes"], power=power,
-            snr=ref["snr"], snrPink=ref["snrPink"], snrFit=ref["snrFit"],
-            snrFitPink=ref["snrFitPink"], raw_power=raw_power, raw_chi2=raw_chi2,
-            possiblePeriodsIndices=cand, possiblePeriods=[float(periods[i]) for i in cand])
+            return out
+        self.period = out.period
+        self.rawDurations = out.rawDurations
+        return out
 
     def periodogram(self, **kwargs):
         """The period grid power() would search (src/gputls/main.py:186-206)."""

@@ -188,6 +188,15 @@ def get_cache(durations, maxwidth_in_samples, per, rp, a, inc, ecc, w, u, limb_d
     width_in_samples and overshoot; rows[k] is the trimmed template flux of duration k.
     """
     ref = reference_transit(maxwidth_in_samples, per, rp, a, inc, ecc, w, u, limb_dark)
+    return cache_rows(durations, maxwidth_in_samples, ref)
+
+
+def cache_rows(durations, maxwidth_in_samples, ref):
+    """get_cache's steps after the reference template `ref` (unit depth, in-transit part
+    only, maxwidth_in_samples points; src/gputls/transit.py:129-160): per duration the
+    fractional-width template at SIGNAL_DEPTH, its width in samples, the row trimmed to
+    the samples below 1 - NUMERICAL_STABILITY_CUTOFF, and overshoot = 1 / (2 - mean / min)
+    of that row."""
     n = np.size(durations)
     overview = np.zeros(n, dtype=[("duration", "f8"), ("width_in_samples", "i8"),
                                   ("overshoot", "f8")])

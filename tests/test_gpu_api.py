@@ -37,7 +37,7 @@ def test_power_recovers_injected_transit(gpu):
     for f in FIELDS:
         assert hasattr(res, f), f
     assert res.period == pytest.approx(truth["period"], rel=2e-3)
-    assert res.SDE > 9
+    assert res.SDE > 14   # 15.01: the oracle-driven pipeline gives the same (test below)
     assert res.depth == pytest.approx(truth["depth"], rel=0.35)
     assert len(res.power) == len(res.periods) == len(res.chi2)
     # transit times are spaced by the period and T0 falls on the injected ephemeris
@@ -60,3 +60,41 @@ def test_power_on_tess_light_curve(gpu):
     assert np.asarray(res.periods).min() <= res.period <= np.asarray(res.periods).max()
     assert len(res.possiblePeriods) > 0
     check_final_chi2(model, res)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32c"])
+def test_power_matches_the_oracle_driven_pipeline(gpu, mode):
+    """power() on the GPU against the same pipeline driven by the FP64 CPU oracle's search
+    (run_power with the oracle in place of the device search, src/gputls/core.py:885-999):
+    full grid, candidates, no-stride re-search, harmonics and the final refinement. FP64:
+    the same candidates, final period, best width and T0 index, and SDE, depth and T0 to
+    1e-9; FP32-compensated: the same final period and SDE within 1e-4."""
+    from gtls_amd.api import run_power
+    from gtls_amd.validate import validate_args
+    t, y, dy, truth = synthetic.single_2min()
+    model = gtls(t, y, dy, verbose=False)
+    res = model.power(mode=mode)
+    plan = model.plan
+    args = validate_args({"mode": mode})
+
+    def oracle_search(idx, skip_point):
+        return oracle.search_plan(plan, skip_point=skip_point, periods=plan.periods[idx],
+                                  k_lo=plan.k_lo[idx], k_hi=plan.k_hi[idx])
+
+    ref = run_power(plan, args, oracle_search)
+    print("power %s: period %.9f / %.9f, SDE %.9f / %.9f, depth %.6g / %.6g, T0 %.9f / %.9f"
+          % (mode, res.period, ref.period, res.SDE, ref.SDE, res.depth, ref.depth, res.T0,
+             ref.T0))
+    assert res.period == ref.period
+    assert abs(res.SDE - ref.SDE) <= 1e-4 * abs(ref.SDE)
+    if mode == "fp64":
+        assert res.possiblePeriodsIndices == ref.possiblePeriodsIndices
+        assert (res.bestWidthIndex, res.bestT0Index) == (ref.bestWidthIndex, ref.bestT0Index)
+        assert res.SDE == pytest.approx(ref.SDE, rel=1e-9)
+        assert res.depth == pytest.approx(ref.depth, rel=1e-9)
+        assert res.T0 == pytest.approx(ref.T0, abs=1e-9)
+        assert res.duration == pytest.approx(ref.duration, abs=1e-12)
+    else:
+        assert res.depth == pytest.approx(ref.depth, rel=1e-4)
+        assert abs(res.T0 - ref.T0) <= 2 * np.median(np.diff(t))
+    assert res.period == pytest.approx(truth["period"], rel=2e-3)

@@ -124,3 +124,49 @@ def test_template_table_invariants():
         assert tb.overshoot[k] == pytest.approx(1 / (2 - np.mean(sig) / np.min(sig)), rel=1e-12)
     # the trimmed rows fill their widths (the template occupies exactly its width)
     assert all(len(r) == d for r, d in zip(tb.rows, w))
+
+
+# --- the resampling, trimming and overshoot steps (src/gputls/transit.py:45-160) fed with
+# analytic reference profiles whose rows and overshoot are known in closed form
+
+
+def test_box_reference_rows_and_overshoot():
+    """A box reference (unit depth everywhere in transit): every row is the box at the
+    signal depth, width_in_samples = int(duration / max * samples) samples long, with
+    overshoot 1 (mean = min)."""
+    from gtls_amd import constants as C
+    from gtls_amd.template import cache_rows
+    samples = 1200
+    durations = np.array([0.01, 0.013, 0.02, 0.035, 0.05])
+    ov, rows = cache_rows(durations, samples, np.zeros(samples))
+    for k, d in enumerate(durations):
+        occ = int(d / durations.max() * samples)
+        assert ov["width_in_samples"][k] == occ
+        assert len(rows[k]) == occ
+        assert np.allclose(rows[k], 1 - C.SIGNAL_DEPTH, rtol=0, atol=1e-15)
+        assert ov["overshoot"][k] == pytest.approx(1.0, abs=1e-15)
+
+
+@pytest.mark.parametrize("h", [40, 125, 300])
+def test_triangle_reference_rows_and_overshoot(h):
+    """A V-shaped reference r(x) = |2 x| on x in [-1/2, 1/2] (unit depth at the centre),
+    sampled on an odd grid so that its linear interpolant is the V itself: the row of
+    2h + 1 occupied samples is 1 - (1 - r_j) D at r_j = |j - h| / h without its two end
+    samples (r = 1, above the trimming cut), so mean(1 - r) = h / (2h - 1) over the row and
+    overshoot = 1 / (2 - (1 - D h / (2h - 1)) / (1 - D))."""
+    from gtls_amd import constants as C
+    from gtls_amd.template import cache_rows
+    samples = 2001
+    ref = np.abs(2 * np.linspace(-0.5, 0.5, samples))
+    occ = 2 * h + 1
+    dmax = 0.1
+    dur = dmax * (occ + 0.5) / samples          # int(dur / dmax * samples) = occ
+    ov, rows = cache_rows(np.array([dur, dmax]), samples, ref)
+    D = C.SIGNAL_DEPTH
+    j = np.arange(1, occ - 1)
+    want = 1 - (1 - np.abs(j - h) / h) * D
+    assert ov["width_in_samples"][0] == occ
+    assert len(rows[0]) == occ - 2
+    assert np.allclose(rows[0], want, rtol=0, atol=1e-12)
+    over = 1 / (2 - (1 - D * h / (2 * h - 1)) / (1 - D))
+    assert ov["overshoot"][0] == pytest.approx(over, rel=1e-12)
