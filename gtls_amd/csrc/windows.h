@@ -309,13 +309,14 @@ __device__ __forceinline__ HSweep<4> h_sweep(const char* __restrict__ X, const c
                                              int64_t st, bool ones = false) {
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
+    const int qr = lane >> 2, qk = lane & 3;   // load lanes: row qr, K slice qk (a_permute)
     HSweep<4> w;
     w.X = X;
     w.C = C;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int64_t row = 64 * st + 16 * t + col;
-        w.xo[t] = (uint32_t)(4 * ((row < rows ? row * q : 0) + 4 * kk));
+        const int64_t row = 64 * st + 16 * t + qr;
+        w.xo[t] = (uint32_t)(4 * ((row < rows ? row * q : 0) + 4 * qk));
     }
     const int32_t ch = col <= F ? col : (ones && col >= kOnesCol ? F + 1 + (col - kOnesCol) : -1);
     w.co = (ch >= 0) ? 4 * (ch * chunk_pad(q) + 4 * kk) : zrel + 16 * kk;
@@ -354,6 +355,27 @@ __device__ __forceinline__ void h_load(const HSweep<NT>& w, int32_t g, int32_t c
     }
 }
 
+// The sample operands are loaded by lanes l = 4 r + s (row r, K slice s: each 4-lane quad
+// reads one row's 64 contiguous bytes, which the L1 cache serves at up to 4x the rate of
+// the operand layout, where the 4 lanes of a quad read 4 different rows) and moved in
+// place to the MFMA operand layout (row l % 16, slice l / 16) right before their MFMAs:
+// lane l takes the 4 dwords of lane 4 (l % 16) + l / 16.
+__device__ __forceinline__ int a_permute_addr() {
+    const int lane = lane_id();
+    return 4 * (4 * (lane & 15) + (lane >> 4));
+}
+template <int NT, bool Ones>
+__device__ __forceinline__ void a_permute(HSet<NT, Ones>& o, int addr) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        uint32_t v[4];
+        __builtin_memcpy(v, &o.a[t], sizeof(v));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v[e]);
+        __builtin_memcpy(&o.a[t], v, sizeof(v));
+    }
+}
+
 // K groups of a sweep over `len` taps (taps past `len` meet zero template entries).
 __device__ __forceinline__ int32_t k_groups(int32_t len) {
     return (len + kKGroup - 1) / kKGroup;
@@ -388,6 +410,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
                                           const HSweep<NT>& cur, const HSweep<NT>& nxt,
                                           int32_t ng, int32_t clo, HSet<NT, Ones>& s0,
                                           HSet<NT, Ones>& s1, int32_t d = 0) {
+    const int paddr = a_permute_addr();
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
@@ -397,6 +420,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
     for (; g + 1 < ng; g += 2) {
         h_load(cur, g + 1, clo, s1, d);
         __builtin_amdgcn_sched_barrier(0);
+        a_permute(s0, paddr);
         h_mma(acc, accw, s0);
         __builtin_amdgcn_sched_barrier(0);
         // group g + 2 of this sweep, or group 0 of the next one (addresses selected, so
@@ -410,10 +434,12 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
         w.co = more ? cur.co : nxt.co;
         h_load(w, more ? g + 2 : 0, clo, s0, d);
         __builtin_amdgcn_sched_barrier(0);
+        a_permute(s1, paddr);
         h_mma(acc, accw, s1);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (g < ng) {   // an odd group count: the last group is in s0
+        a_permute(s0, paddr);
         h_mma(acc, accw, s0);
         __builtin_amdgcn_sched_barrier(0);
         h_load(nxt, 0, clo, s0, d);
@@ -587,14 +613,15 @@ __device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __res
     // kToepStep st + 256 t + 16 col .. + 15 (rows wholly past the last candidate read
     // sample 0, results unused); B operand: template pairs at taps 4 kk .. + 3 shifted by
     // q col
+    const int qr = lane >> 2, qk = lane & 3;   // load lanes: row qr, K slice qk (a_permute)
     auto sweep = [&](const char* Xp, const char* Tp, int64_t st) {
         HSweep<NT> w;
         w.X = Xp;
         w.C = Tp;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int64_t m0 = kToepStep * st + 256 * t + 16 * col;
-            w.xo[t] = (uint32_t)(4 * ((m0 < K ? m0 * q : 0) + 4 * kk));
+            const int64_t m0 = kToepStep * st + 256 * t + 16 * qr;
+            w.xo[t] = (uint32_t)(4 * ((m0 < K ? m0 * q : 0) + 4 * qk));
         }
         w.co = 4 * (4 * kk - q * col);
         return w;
