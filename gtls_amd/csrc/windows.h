@@ -236,6 +236,70 @@ struct HalfVW {
     __device__ __forceinline__ HalfVW operator+(int64_t o) const { return HalfVW{v + o, w + o}; }
 };
 
+
+// The running first minimum of one lane over its candidates of one width in the matrix-core
+// paths. A lane visits its candidates in increasing position order, so a later candidate
+// replaces the minimum only when its statistic is strictly smaller. thr bounds from above
+// (statistic - chi2_flat) of the minimum: a candidate whose float estimate of that
+// difference, less the estimate's error bound, is not below thr cannot replace it, and its
+// exact float64 statistic is skipped (NaN estimates are evaluated exactly). The result is
+// the same as evaluating every candidate exactly.
+struct LaneMin {
+    ArgMin best;
+    float thr;
+    bool sent_live;   // the sentinel N is below the minimum: a closed gate replaces it
+};
+
+__device__ __forceinline__ LaneMin lane_min_init() {
+    return LaneMin{ArgMin{__builtin_inf(), 0x7fffffff, 0x7fffffff}, __builtin_inff(), true};
+}
+
+__device__ __forceinline__ void lane_take(LaneMin& L, double v, int32_t s, const WidthArgs& a) {
+    L.best = ArgMin{v, a.k, s};
+    const float dv = (float)(v - a.chi2f);
+    L.thr = dv + 1e-6f * fabsf(dv) + 1e-12f * (float)fabs(a.chi2f) + 1e-30f;
+    L.sent_live = a.sentinel < v;
+}
+
+// Float copies of one width's constants for the estimates.
+struct WidthConstsF {
+    float wc, ssk, os2, inv_d, depth_min;
+};
+__device__ __forceinline__ WidthConstsF width_consts_f(const WidthArgs& a) {
+    return WidthConstsF{(float)a.wc, (float)a.ssk, (float)a.os2, (float)a.inv_d,
+                        (float)a.depth_min};
+}
+
+// One candidate at position s with an open (gate) or closed window-mean gate, r = 2 m
+// overshoot (rs, float), and window sums sa (A) and, with per-sample weights, sb (B):
+// statistic chi2_flat + w0 (r^2 sum S^2 - 2 r A) (common weight w0, float64) or
+// chi2_flat + (r^2 B - 2 r A) (float), else the sentinel N.
+template <bool ConstW>
+__device__ __forceinline__ void tile_candidate(LaneMin& L, const WidthArgs& a,
+                                               const WidthConstsF& f, bool gate, float rs,
+                                               float sa, float sb, int32_t s) {
+    if (gate) {
+        if constexpr (ConstW) {
+            const float t1 = f.wc * (rs * rs) * f.ssk;
+            const float t2 = 2.f * f.wc * rs * sa;
+            const float lo = (t1 - t2) - 1e-6f * (fabsf(t1) + fabsf(t2));
+            if (!(lo >= L.thr)) {
+                const double v =
+                    a.chi2f + a.wc * ((double)(rs * rs) * a.ssk - 2.0 * (double)rs * (double)sa);
+                if (v < L.best.v) lane_take(L, v, s, a);
+            }
+        } else {
+            const float df = rs * rs * sb - 2.f * rs * sa;
+            if (!(df >= L.thr)) {
+                const double v = a.chi2f + (double)df;
+                if (v < L.best.v) lane_take(L, v, s, a);
+            }
+        }
+    } else if (L.sent_live) {
+        lane_take(L, a.sentinel, s, a);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Matrix-core window sums (float32 search). For stride q and width d = F q + rem, the
 // window of candidate k covers data blocks k .. k + F (block j = samples jq .. jq + q - 1):
@@ -499,7 +563,9 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     const int32_t F = d / q;
     const int64_t rows = K + F;
     const int64_t nsuper = (rows + 63) / 64;
-    ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
+    LaneMin L = lane_min_init();
+    const WidthConstsF fc = width_consts_f(a);
+    const float inv_xd = P.inv_x * fc.inv_d;
     const int32_t ng = chunk_pad(q) / kKGroup;
     int ring0 = 0;   // ring row of G row 64 T
     // sweeps in order (A plane, then B plane, of each super-tile); the first group of each
@@ -562,20 +628,17 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
         wave_sync_lds();
         if (cand) {
             const int32_t s = (int32_t)(kc * q);
-            const double m = TileMean ? (double)(su * P.inv_x) * a.inv_d : (dhi - dlo) * a.inv_d;
-            if (m > a.depth_min) {
-                const float rs = (float)(m * a.os2);
-                const double delta =
-                    ConstW ? a.wc * ((double)(rs * rs) * a.ssk - 2.0 * (double)rs * (double)sa)
-                           : (double)(rs * rs * sb - 2.f * rs * sa);
-                consider(best, a.chi2f + delta, a.k, s);
+            if constexpr (TileMean) {
+                const float m = su * inv_xd;
+                tile_candidate<ConstW>(L, a, fc, m > fc.depth_min, m * fc.os2, sa, sb, s);
             } else {
-                consider(best, a.sentinel, a.k, s);
+                const double m = (dhi - dlo) * a.inv_d;
+                tile_candidate<ConstW>(L, a, fc, m > a.depth_min, (float)(m * a.os2), sa, sb, s);
             }
         }
         ring0 = ring0 + 64 >= kRingRows ? ring0 + 64 - kRingRows : ring0 + 64;
     }
-    return best;
+    return L.best;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -608,7 +671,9 @@ __device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __res
     const int64_t K = a.K;
     const int32_t ng = k_groups(d + 15 * q);         // d + 15 q taps
     const int64_t nsuper = (K + kToepStep - 1) / kToepStep;
-    ArgMin best{__builtin_inf(), 0x7fffffff, 0x7fffffff};
+    LaneMin L = lane_min_init();
+    const WidthConstsF fc = width_consts_f(a);
+    const float inv_xd = P.inv_x * fc.inv_d;
     // sweep of step st over one plane: row `col` of tile t covers candidates
     // kToepStep st + 256 t + 16 col .. + 15 (rows wholly past the last candidate read
     // sample 0, results unused); B operand: template pairs at taps 4 kk .. + 3 shifted by
@@ -664,24 +729,20 @@ __device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __res
                 const int64_t m = kToepStep * st + 256 * t + 16 * (4 * kk + v) + col;
                 if (m < K) {
                     const int32_t s = (int32_t)(m * q);
-                    double mm;
-                    if constexpr (ConstW) mm = (double)(acc2[t][v] * P.inv_x) * a.inv_d;
-                    else mm = (dhi[e] - (s > 0 ? dlo[e] : 0.0)) * a.inv_d;
-                    if (mm > a.depth_min) {
-                        const float rs = (float)(mm * a.os2);
-                        const float sa = acc[t][v] * P.inv_x;
-                        const double delta =
-                            ConstW ? a.wc * ((double)(rs * rs) * a.ssk - 2.0 * (double)rs * (double)sa)
-                                   : (double)(rs * rs * (acc2[t][v] * P.inv_w) - 2.f * rs * sa);
-                        consider(best, a.chi2f + delta, a.k, s);
+                    const float sa = acc[t][v] * P.inv_x;
+                    if constexpr (ConstW) {
+                        const float mm = acc2[t][v] * inv_xd;
+                        tile_candidate<true>(L, a, fc, mm > fc.depth_min, mm * fc.os2, sa, 0.f, s);
                     } else {
-                        consider(best, a.sentinel, a.k, s);
+                        const double mm = (dhi[e] - (s > 0 ? dlo[e] : 0.0)) * a.inv_d;
+                        tile_candidate<false>(L, a, fc, mm > a.depth_min, (float)(mm * a.os2), sa,
+                                              acc2[t][v] * P.inv_w, s);
                     }
                 }
             }
         }
     }
-    return best;
+    return L.best;
 }
 
 }  // namespace gtls
