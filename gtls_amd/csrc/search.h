@@ -279,6 +279,16 @@ __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw
     carry = total;
 }
 
+// The samples of one sort window in float32 with a common weight when no running sums are
+// wanted: position o0 + j takes the (hi, lo) pair of sx u of sample sidx[j]. All threads
+// call it; no barrier.
+__device__ __forceinline__ void gather_window_pairs(const float2* __restrict__ uw, int64_t o0,
+                                                    int cnt, const int32_t* sidx, uint32_t* hb,
+                                                    float sx) {
+    for (int j = threadIdx.x; j < cnt; j += kBlock)
+        __builtin_nontemporal_store(split_pair(uw[sidx[j]].x * sx), &hb[o0 + j]);
+}
+
 // Stage 1 + 2 of each period of the chunk, one workgroup per period (persistent):
 //   1. fold + stable phase sort (fold_sort.h)                  -> perm
 //   2. gather in phase order with the wrap-around extension    -> X, running sum D
@@ -362,6 +372,11 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
                 p.t + pt0, n, P, p.negative_time != 0, perm, hist, red_i, s_counts, s_phi,
                 s_idx, s_mids, s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                     if constexpr (sizeof(Real) == 4) {
+                        if (constw && !D) {
+                            gather_window_pairs((const float2*)uw, wstart, wcnt, s_idx, sink.hb,
+                                                sink.sx);
+                            return;
+                        }
                         if (constw) {
                             gather_scan_window((const float2*)uw, wstart, wcnt, s_idx, s_phi,
                                                sink.hb, sink.sx, D, carry, red_d);
