@@ -115,9 +115,9 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
         L.cplen = L.n_chunked + L.n_zero;
     }
     L.tmplh = off;
-    off = align_up(off + 20 * L.tplen);
+    off = align_up(off + 8 * L.tplen);
     L.tmplc = off;
-    off = align_up(off + 16 * L.cplen);
+    off = align_up(off + 8 * L.cplen);
     L.xsc = off;
     off = align_up(off + 8 * (int64_t)b->n_curves);
     L.ss = off;

@@ -53,13 +53,11 @@ struct SearchParams {
     const R2* tmpl2;        // per template sample (S, S^2)
     const Real* tmpl1;      // per template sample S
     const Real* tmplsq;     // per template sample S^2
-    // float32 matrix-core operands in float16 pair planes (windows.h):
-    //   tmplh: template rows with kTmplPad zeros on both sides, planes of S (hi, hi) and
-    //          (lo, 0) pairs and of S^2 (hi, hi) and (lo, 0) pairs, each tlo bytes after
-    //          the previous one (the Toeplitz tiles);
-    //   tmplc: chunked template rows of the diagonal-sum tiles in the same four planes,
-    //          each clo bytes after the previous one and ending in a zero block at chunk
-    //          offset zoff
+    // float32 matrix-core operands in float16 (hi, lo) pair planes (windows.h):
+    //   tmplh: template rows with kTmplPad zeros on both sides, a plane of S pairs and,
+    //          tlo bytes after it, one of S^2 pairs (the Toeplitz tiles);
+    //   tmplc: chunked template rows of the diagonal-sum tiles in the same two planes,
+    //          clo bytes apart, each ending in a zero block at chunk offset zoff
     const char* tmplh;
     const char* tmplc;
     int32_t tlo, clo;
@@ -507,9 +505,9 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
             const int64_t toff = uniform(p.tmpl_off[wg] + 2 * kTmplPad * wg + kTmplPad);
             const char* T = p.tmplh + 4 * toff;
             if (constw)
-                w = toep_width<true>(hp, T, nullptr, p.tlo, D, a);
+                w = toep_width<true>(hp, T, nullptr, D, a);
             else
-                w = toep_width<false>(hp, T, T + 2 * (int64_t)p.tlo, p.tlo, D, a);
+                w = toep_width<false>(hp, T, T + p.tlo, D, a);
             if (argmin_less(w, best)) best = w;
             return best;
         }
@@ -518,11 +516,11 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
             const char* C = p.tmplc + 4 * co;
             const int32_t zrel = (int32_t)(4 * (p.zoff - co));
             if (constw && F + 1 < kOnesCol)
-                w = mfma_width<true, true>(hp, C, nullptr, p.clo, zrel, D, a);
+                w = mfma_width<true, true>(hp, C, nullptr, zrel, D, a);
             else if (constw)
-                w = mfma_width<true, false>(hp, C, nullptr, p.clo, zrel, D, a);
+                w = mfma_width<true, false>(hp, C, nullptr, zrel, D, a);
             else
-                w = mfma_width<false, false>(hp, C, C + 2 * (int64_t)p.clo, p.clo, zrel, D, a);
+                w = mfma_width<false, false>(hp, C, C + p.clo, zrel, D, a);
             if (argmin_less(w, best)) best = w;
             return best;
         }
@@ -671,39 +669,32 @@ __global__ void prep_points_kernel(const double* y, const double* dy, int64_t n,
     uw[i] = o;
 }
 
-// Template value sv at pair index i of the four float16 pair planes from th (plen pairs
-// each): S (hi, hi), S (lo, 0), S^2 (hi, hi), S^2 (lo, 0), S rounded to float32 first.
+// Template value sv at pair index i of the two float16 (hi, lo) pair planes from th (plen
+// pairs each): S and S^2, S rounded to float32 first.
 __device__ __forceinline__ void put_template_pairs(uint32_t* th, int64_t plen, int64_t i,
                                                    double sv) {
-    const float s1 = (float)sv, s2 = (float)(sv * sv);
-    const _Float16 h1 = (_Float16)s1, h2 = (_Float16)s2;
-    th[i] = f16_pair(h1, h1);
-    th[i + plen] = f16_pair((_Float16)(s1 - (float)h1), (_Float16)0.f);
-    th[i + 2 * plen] = f16_pair(h2, h2);
-    th[i + 3 * plen] = f16_pair((_Float16)(s2 - (float)h2), (_Float16)0.f);
+    const float s1 = (float)sv;
+    th[i] = split_pair(s1);
+    th[i + plen] = split_pair((float)(sv * sv));
 }
 
 // Search-precision template rows, (S, S^2) pairs, S alone and S^2 alone, each row surrounded by
 // kTmplPad zeros: row w starts at tmpl_off[w] + 2 kTmplPad w + kTmplPad. One block per row.
-// Float32: also the float16 pair planes of S and S^2 (S rounded to float32 first) for the
-// Toeplitz tiles, five planes of plen pairs from th: S (hi, hi), S (lo, 0), S^2 (hi, hi),
-// S^2 (lo, 0), and (1, 1) over the row's d samples (zero in the padding).
+// Float32: also the float16 (hi, lo) pair planes of S and S^2 (S rounded to float32 first)
+// for the Toeplitz tiles, two planes of plen pairs from th.
 template <class Real>
 __global__ void prep_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
                                      int64_t nw, int64_t total_tmpl, int64_t tail,
                                      typename Pair<Real>::type* out, Real* out1, Real* outsq,
                                      uint32_t* th, int64_t plen) {
-    auto put = [&](int64_t i, double sv, bool one) {
+    auto put = [&](int64_t i, double sv) {
         typename Pair<Real>::type o;
         o.x = (Real)sv;
         o.y = (Real)(sv * sv);
         out[i] = o;
         out1[i] = (Real)sv;
         outsq[i] = (Real)(sv * sv);
-        if (th) {
-            put_template_pairs(th, plen, i, sv);
-            th[i + 4 * plen] = one ? f16_pair((_Float16)1.f, (_Float16)1.f) : 0u;
-        }
+        if (th) put_template_pairs(th, plen, i, sv);
     };
     const int64_t w = blockIdx.x;
     if (w >= nw) {
@@ -711,7 +702,7 @@ __global__ void prep_template_kernel(const double* S, const int64_t* off, const 
         const int64_t end = total_tmpl + 2 * kTmplPad * nw;
         for (int64_t i = end + (int64_t)(w - nw) * blockDim.x + threadIdx.x;
              i < end + tail; i += (int64_t)(gridDim.x - nw) * blockDim.x)
-            put(i, 0.0, false);
+            put(i, 0.0);
         return;
     }
     const int32_t d = max(widths[w], 0);
@@ -719,15 +710,14 @@ __global__ void prep_template_kernel(const double* S, const int64_t* off, const 
     const int64_t dst = off[w] + 2 * kTmplPad * w;   // start of the padded row
     for (int32_t i = threadIdx.x; i < d + 2 * kTmplPad; i += blockDim.x) {
         const int32_t j = i - kTmplPad;
-        const bool in = j >= 0 && j < d;
-        put(dst + i, in ? S[src + j] : 0.0, in);
+        put(dst + i, (j >= 0 && j < d) ? S[src + j] : 0.0);
     }
 }
 
-// Chunked template rows of the diagonal-sum tiles (float32) in the four pair planes of
+// Chunked template rows of the diagonal-sum tiles (float32) in the two pair planes of
 // plen pairs from c (as put_template_pairs), each ending in a zero block of nz pairs at
 // zoff; after the F + 1 template chunks, the ones chunks F + 1 (b < q) and F + 2
-// (b < d - F q), as (1, 1) and (0, 0) pairs. One block per width; widths those tiles do
+// (b < d - F q): the pair of 1 in the S plane, 0 in the S^2 plane. One block per width; widths those tiles do
 // not take (q < 4 or more than 16 chunks) get no row. The extra blocks past the last width
 // clear the zero blocks.
 __global__ void chunk_template_kernel(const double* S, const int64_t* off, const int32_t* widths,
@@ -748,7 +738,7 @@ __global__ void chunk_template_kernel(const double* S, const int64_t* off, const
     const int32_t qp = chunk_pad(q);
     const int64_t co = chunk_offset(off[w], w);
     const int32_t total = (F + 3) * qp;
-    const uint32_t one = f16_pair((_Float16)1.f, (_Float16)1.f);
+    const uint32_t one = split_pair(1.f);
     for (int32_t i = threadIdx.x; i < total; i += blockDim.x) {
         const int32_t ch = i / qp, b = i - ch * qp;
         if (ch <= F) {
@@ -756,7 +746,8 @@ __global__ void chunk_template_kernel(const double* S, const int64_t* off, const
             put_template_pairs(c, plen, co + i, b < len ? S[off[w] + (int64_t)ch * q + b] : 0.0);
         } else {
             const bool in = b < (ch == F + 1 ? q : rem);
-            for (int k = 0; k < 4; ++k) c[co + i + k * plen] = (in && k % 2 == 0) ? one : 0u;
+            c[co + i] = in ? one : 0u;
+            c[co + i + plen] = 0u;
         }
     }
 }

@@ -186,9 +186,10 @@ namespace gtls {
 // ---------------------------------------------------------------------------------------
 // Float32 search operands in float16 high/low pairs. A value x (scaled by a power of two
 // so that |x| < 2^14) is stored as the pair (hi, lo) = (f16(x), f16(x - hi)) in one
-// 32-bit word; hi + lo holds x to about 2^-22 relative. Sample planes are arrays of such
-// pairs. A template value s is stored in two pair planes, (hi(s), hi(s)) and (lo(s), 0),
-// so that with the K dimension running over (sample, half) the two products
+// 32-bit word; hi + lo holds x to about 2^-22 relative. Sample and template planes are
+// arrays of such pairs. From a template pair (hi(s), lo(s)) the registers form the two
+// operands (hi(s), hi(s)) and (lo(s), 0) (one byte permute and one shift), so that with
+// the K dimension running over (sample, half) the two products
 //     sum_i hi(x_i) hi(s_i) + lo(x_i) hi(s_i)   and   sum_i hi(x_i) lo(s_i)
 // are one v_mfma_f32_16x16x32_f16 each per 16 samples (the dropped lo lo term is ~2^-22
 // of the product). Every operand is 4 consecutive pairs (16 bytes) starting at a pair
@@ -340,9 +341,9 @@ __device__ __forceinline__ int ring_wrap(int r) {
 // Chunked template rows of the matrix-core path: for width w with stride q, F = d / q and
 // q_pad = q rounded up to whole K groups, chunk c <= F sits at chunk_offset(w) + c q_pad
 // (in pairs of each plane), holds S[c q + b] for b < its length (q, or d - F q for c = F)
-// and zeros up to q_pad; chunks F + 1 and F + 2 hold ones for b < q and b < d - F q (the
-// window sums of the samples themselves, in tile columns 14 and 15 when F <= 13). Other
-// columns read the zero block that follows the rows of each plane.
+// and zeros up to q_pad; in the S plane, chunks F + 1 and F + 2 hold ones for b < q and
+// b < d - F q (the window sums of the samples themselves, in tile columns 14 and 15 when
+// F <= 13). Other columns read the zero block that follows the rows of each plane.
 // (F + 3) q_pad <= 4 d + 270, so the offsets below leave room for every row.
 constexpr int kOnesCol = 14;     // tile column of the ones chunk (F + 1); the rem ones in 15
 __host__ __device__ inline int64_t chunk_offset(int64_t tmpl_off, int64_t w) {
@@ -352,13 +353,12 @@ __host__ __device__ inline int32_t chunk_pad(int32_t q) {
     return (q + kKGroup - 1) / kKGroup * kKGroup;
 }
 
-// Operand byte offsets of one sweep over NT tiles: the sample pair plane X and the first
-// template pair plane C ((hi, hi) pairs; the (lo, 0) pairs sit clo bytes further), the
-// lane's A offsets (one per tile) and B offset.
+// Operand byte offsets of one sweep over NT tiles: the sample pair plane X and the
+// template pair plane C, the lane's A offsets (one per tile) and B offset.
 template <int NT>
 struct HSweep {
     const char* X;       // sample pair plane (wave-uniform)
-    const char* C;       // template (hi, hi) pair plane (wave-uniform)
+    const char* C;       // template pair plane (wave-uniform)
     uint32_t xo[NT];     // lane's A operand byte offsets, one per tile
     int32_t co;          // lane's B operand byte offset
 };
@@ -387,8 +387,8 @@ __device__ __forceinline__ HSweep<4> h_sweep(const char* __restrict__ X, const c
     return w;
 }
 
-// One K group's operands of a sweep: the sample pairs of the NT tiles and the two
-// template pair operands.
+// One K group's operands of a sweep: the sample pairs of the NT tiles and the template
+// pairs (expanded into the two MFMA operands by h_mma).
 // With Ones, also a third template operand: (1, 1) pairs where the lane's taps fall inside
 // the window of its column (a row of d ones shifted like the template: the window sums of
 // the samples, Toeplitz tiles), formed in registers from the lane's template tap
@@ -396,19 +396,17 @@ __device__ __forceinline__ HSweep<4> h_sweep(const char* __restrict__ X, const c
 template <int NT, bool Ones = false>
 struct HSet {
     h8 a[NT];
-    h8 b1, b2;
+    h8 b;
     h8 b3[Ones ? 1 : 0];
 };
 
 template <int NT, bool Ones>
-__device__ __forceinline__ void h_load(const HSweep<NT>& w, int32_t g, int32_t clo,
-                                       HSet<NT, Ones>& o, int32_t d = 0) {
+__device__ __forceinline__ void h_load(const HSweep<NT>& w, int32_t g, HSet<NT, Ones>& o,
+                                       int32_t d = 0) {
     const int32_t off = 4 * kKGroup * g;
 #pragma unroll
     for (int t = 0; t < NT; ++t) o.a[t] = ld_h8(w.X + (w.xo[t] + (uint32_t)off));
-    const char* c = w.C + (int64_t)(w.co + off);
-    o.b1 = ld_h8(c);
-    o.b2 = ld_h8(c + clo);
+    o.b = ld_h8(w.C + (int64_t)(w.co + off));
     if constexpr (Ones) {
         const int32_t tap = (w.co + off) / 4;   // exact: w.co and off are multiples of 4
         uint32_t v[4];
@@ -451,12 +449,23 @@ __device__ __forceinline__ int32_t k_groups(int32_t len) {
 template <int NT, bool Ones>
 __device__ __forceinline__ void h_mma(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
                                       const HSet<NT, Ones>& o) {
+    // template operands (hi, hi) and (lo, 0) from the (hi, lo) pairs
+    uint32_t p[4], v1[4], v2[4];
+    __builtin_memcpy(p, &o.b, sizeof(p));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        v1[e] = __builtin_amdgcn_perm(p[e], p[e], 0x01000100u);
+        v2[e] = p[e] >> 16;
+    }
+    h8 b1, b2;
+    __builtin_memcpy(&b1, v1, sizeof(v1));
+    __builtin_memcpy(&b2, v2, sizeof(v2));
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(o.a[t], o.b1, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(o.a[t], b1, acc[t], 0, 0, 0);
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(o.a[t], o.b2, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(o.a[t], b2, acc[t], 0, 0, 0);
     if constexpr (Ones) {
 #pragma unroll
         for (int t = 0; t < NT; ++t)
@@ -472,7 +481,7 @@ __device__ __forceinline__ void h_mma(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
 template <int NT, bool Ones>
 __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
                                           const HSweep<NT>& cur, const HSweep<NT>& nxt,
-                                          int32_t ng, int32_t clo, HSet<NT, Ones>& s0,
+                                          int32_t ng, HSet<NT, Ones>& s0,
                                           HSet<NT, Ones>& s1, int32_t d = 0) {
     const int paddr = a_permute_addr();
 #pragma unroll
@@ -482,7 +491,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
     }
     int32_t g = 0;
     for (; g + 1 < ng; g += 2) {
-        h_load(cur, g + 1, clo, s1, d);
+        h_load(cur, g + 1, s1, d);
         __builtin_amdgcn_sched_barrier(0);
         a_permute(s0, paddr);
         h_mma(acc, accw, s0);
@@ -496,7 +505,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 #pragma unroll
         for (int t = 0; t < NT; ++t) w.xo[t] = more ? cur.xo[t] : nxt.xo[t];
         w.co = more ? cur.co : nxt.co;
-        h_load(w, more ? g + 2 : 0, clo, s0, d);
+        h_load(w, more ? g + 2 : 0, s0, d);
         __builtin_amdgcn_sched_barrier(0);
         a_permute(s1, paddr);
         h_mma(acc, accw, s1);
@@ -506,19 +515,19 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
         a_permute(s0, paddr);
         h_mma(acc, accw, s0);
         __builtin_amdgcn_sched_barrier(0);
-        h_load(nxt, 0, clo, s0, d);
+        h_load(nxt, 0, s0, d);
     }
 }
 
 // The sweep `cur` of a super-tile, written to the ring (see run_sweep).
 __device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSweep<4>& nxt,
-                                                int32_t ng, int32_t clo, HSet<4>& s0,
+                                                int32_t ng, HSet<4>& s0,
                                                 int ring0, float* ring) {
     const int lane = lane_id();
     const int kk = lane >> 4, col = lane & 15;
     mfma_v4 acc[4], unused[4];
     HSet<4> s1;
-    run_sweep<4, false>(acc, unused, cur, nxt, ng, clo, s0, s1);
+    run_sweep<4, false>(acc, unused, cur, nxt, ng, s0, s1);
     // rows 16 t + 4 kk .. + 3 of column col: one 16-byte store (ring0 is a multiple of 16
     // and kRingRows of 4, so the four rows never wrap apart)
 #pragma unroll
@@ -543,15 +552,14 @@ struct HPlanes {
 
 // Every candidate of one (period, width) item on the matrix cores, returning the lane's
 // first minimum of the statistic. C, C2: chunked template pair planes of S and S^2
-// ((hi, hi) pairs; (lo, 0) pairs clo bytes further; their zero blocks zrel bytes from C
-// and C2). Reads of X stay below N + d + q + 16 (zero-filled past the gathered samples).
+// (their zero blocks zrel bytes from C and C2). Reads of X stay below N + d + q + 16 (zero-filled past the gathered samples).
 // The window means: with TileMean (common weight, F <= 13), from the ones columns of the
 // same tiles, the window sum of u = sum_{c<F} G(k + c, 14) + G(k + F, 15) in float32;
 // otherwise from D, the float64 prefix sums of u.
 template <bool ConstW, bool TileMean>
 __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __restrict__ C,
-                                             const char* __restrict__ C2, int32_t clo,
-                                             int32_t zrel, const double* __restrict__ D,
+                                             const char* __restrict__ C2, int32_t zrel,
+                                             const double* __restrict__ D,
                                              WidthArgs a) {
     static_assert(ConstW || !TileMean, "tile means need the samples u");
     const int lane = lane_id();
@@ -571,7 +579,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     // sweeps in order (A plane, then B plane, of each super-tile); the first group of each
     // sweep is loaded during the one before (past the last super-tile: row block 0 again)
     HSet<4> ops;
-    h_load(h_sweep(P.X, C, zrel, q, F, rows, 0, TileMean), 0, clo, ops);
+    h_load(h_sweep(P.X, C, zrel, q, F, rows, 0, TileMean), 0, ops);
     for (int64_t st = 0; st < nsuper; ++st) {
         const int64_t kc = 64 * st - F + lane;
         const bool cand = kc >= 0 && kc < K;
@@ -587,12 +595,12 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
         const HSweep<4> wa = h_sweep(P.X, C, zrel, q, F, rows, st, TileMean);
         if constexpr (ConstW) {
-            super_tile_rows(wa, h_sweep(P.X, C, zrel, q, F, rows, stn, TileMean), ng, clo, ops,
+            super_tile_rows(wa, h_sweep(P.X, C, zrel, q, F, rows, stn, TileMean), ng, ops,
                             ring0, ringA);
         } else {
             const HSweep<4> wb = h_sweep(P.W, C2, zrel, q, F, rows, st);
-            super_tile_rows(wa, wb, ng, clo, ops, ring0, ringA);
-            super_tile_rows(wb, h_sweep(P.X, C, zrel, q, F, rows, stn), ng, clo, ops, ring0,
+            super_tile_rows(wa, wb, ng, ops, ring0, ringA);
+            super_tile_rows(wb, h_sweep(P.X, C, zrel, q, F, rows, stn), ng, ops, ring0,
                             ringB);
         }
         wave_sync_lds();
@@ -650,8 +658,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
 // + l % 16. NT tiles (256 NT candidates, a step) share each template operand. The
 // template rows carry kTmplPad (>= 15 q + 16) zeros on both sides, so taps past a shifted
 // template read zeros, and the sample plane is zero past the data (reads stay below
-// N + d + 15 q + 16). T, T2: template pair planes of S and S^2 ((hi, hi) pairs; the
-// (lo, 0) pairs tlo bytes further). With a common weight the window sums of u come from
+// N + d + 15 q + 16). T, T2: template pair planes of S and S^2. With a common weight the window sums of u come from
 // the same tiles with a row of d ones (formed in registers, HSet) in float32; with
 // per-sample weights from D, the float64 prefix sums of u.
 // ---------------------------------------------------------------------------------------
@@ -662,7 +669,7 @@ constexpr int kToepStep = 256 * kToepTiles;   // candidates per step
 
 template <bool ConstW>
 __device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __restrict__ T,
-                                             const char* __restrict__ T2, int32_t tlo,
+                                             const char* __restrict__ T2,
                                              const double* __restrict__ D, WidthArgs a) {
     constexpr int NT = kToepTiles;
     const int lane = lane_id();
@@ -693,7 +700,7 @@ __device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __res
     };
     mfma_v4 acc[NT], acc2[NT];
     HSet<NT, ConstW> s0, s1;
-    h_load(sweep(P.X, T, 0), 0, tlo, s0, d);
+    h_load(sweep(P.X, T, 0), 0, s0, d);
     for (int64_t st = 0; st < nsuper; ++st) {
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
         // the window-mean prefix sums D[s + d - 1] and D[s - 1] (0 at s = 0) of the lane's
@@ -712,12 +719,12 @@ __device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __res
         // A sums (and the sums of u with a common weight, the B sums with the weights); the
         // first group of the next sweep is loaded during this one
         if constexpr (ConstW) {
-            run_sweep<NT, true>(acc, acc2, sweep(P.X, T, st), sweep(P.X, T, stn), ng, tlo, s0, s1,
+            run_sweep<NT, true>(acc, acc2, sweep(P.X, T, st), sweep(P.X, T, stn), ng, s0, s1,
                                 d);
         } else {
-            run_sweep<NT, false>(acc, acc2, sweep(P.X, T, st), sweep(P.W, T2, st), ng, tlo, s0, s1);
+            run_sweep<NT, false>(acc, acc2, sweep(P.X, T, st), sweep(P.W, T2, st), ng, s0, s1);
             mfma_v4 unused[NT];
-            run_sweep<NT, false>(acc2, unused, sweep(P.W, T2, st), sweep(P.X, T, stn), ng, tlo, s0,
+            run_sweep<NT, false>(acc2, unused, sweep(P.W, T2, st), sweep(P.X, T, stn), ng, s0,
                                  s1);
         }
         // the statistic of the lane's 4 NT candidates
