@@ -515,12 +515,23 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
             const int64_t co = uniform(chunk_offset(p.tmpl_off[wg], wg));
             const char* C = p.tmplc + 4 * co;
             const int32_t zrel = (int32_t)(4 * (p.zoff - co));
-            if (constw && F + 1 < kOnesCol)
-                w = mfma_width<true, true>(hp, C, nullptr, zrel, D, a);
-            else if (constw)
-                w = mfma_width<true, false>(hp, C, nullptr, zrel, D, a);
-            else
-                w = mfma_width<false, false>(hp, C, C + p.clo, zrel, D, a);
+            // template operands staged in LDS when they fit
+            const int32_t ng = chunk_pad(q) / kKGroup;
+            if (constw && F + 1 < kOnesCol) {
+                if (ng <= kStageGroupsConstW)
+                    w = mfma_width<true, true, true>(hp, C, nullptr, zrel, D, a);
+                else
+                    w = mfma_width<true, true, false>(hp, C, nullptr, zrel, D, a);
+            } else if (constw) {
+                if (ng <= kStageGroupsConstW)
+                    w = mfma_width<true, false, true>(hp, C, nullptr, zrel, D, a);
+                else
+                    w = mfma_width<true, false, false>(hp, C, nullptr, zrel, D, a);
+            } else if (ng <= kStageGroupsWeighted) {
+                w = mfma_width<false, false, true>(hp, C, C + p.clo, zrel, D, a);
+            } else {
+                w = mfma_width<false, false, false>(hp, C, C + p.clo, zrel, D, a);
+            }
             if (argmin_less(w, best)) best = w;
             return best;
         }

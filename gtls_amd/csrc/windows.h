@@ -333,6 +333,9 @@ __device__ __forceinline__ h8 ld_h8(const char* p) {
     return r;
 }
 
+// LDS bytes (address space 3: ds_read / ds_write).
+typedef __attribute__((address_space(3))) char lds_char;
+
 __device__ __forceinline__ int ring_wrap(int r) {
     r = r < 0 ? r + kRingRows : r;
     return r >= kRingRows ? r - kRingRows : r;
@@ -354,13 +357,16 @@ __host__ __device__ inline int32_t chunk_pad(int32_t q) {
 }
 
 // Operand byte offsets of one sweep over NT tiles: the sample pair plane X and the
-// template pair plane C, the lane's A offsets (one per tile) and B offset.
+// template pair plane C, the lane's A offsets (one per tile) and B offset; or, for B
+// operands staged in LDS (h_load with BL), the lane's B operand of K group 0 at LB, group
+// g 1024 g bytes further.
 template <int NT>
 struct HSweep {
     const char* X;       // sample pair plane (wave-uniform)
     const char* C;       // template pair plane (wave-uniform)
     uint32_t xo[NT];     // lane's A operand byte offsets, one per tile
     int32_t co;          // lane's B operand byte offset
+    lds_char* LB;        // lane's staged B operand of group 0 (BL)
 };
 
 // The diagonal-sum sweep of super-tile st (64 G rows): A operand X[row q + b]: samples
@@ -400,13 +406,14 @@ struct HSet {
     h8 b3[Ones ? 1 : 0];
 };
 
-template <int NT, bool Ones>
+template <int NT, bool Ones, bool BL = false>
 __device__ __forceinline__ void h_load(const HSweep<NT>& w, int32_t g, HSet<NT, Ones>& o,
                                        int32_t d = 0) {
     const int32_t off = 4 * kKGroup * g;
 #pragma unroll
     for (int t = 0; t < NT; ++t) o.a[t] = ld_h8(w.X + (w.xo[t] + (uint32_t)off));
-    o.b = ld_h8(w.C + (int64_t)(w.co + off));
+    if constexpr (BL) o.b = *(const __attribute__((address_space(3))) h8*)(w.LB + 1024 * g);
+    else o.b = ld_h8(w.C + (int64_t)(w.co + off));
     if constexpr (Ones) {
         const int32_t tap = (w.co + off) / 4;   // exact: w.co and off are multiples of 4
         uint32_t v[4];
@@ -478,7 +485,7 @@ __device__ __forceinline__ void h_mma(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
 // does next. Two explicit operand sets with straight-line pairs of groups: each MFMA
 // block waits only for its own set while the other set's loads are in flight (the
 // scheduling barriers keep each load block ahead of the MFMA block before it).
-template <int NT, bool Ones>
+template <int NT, bool Ones, bool BL = false>
 __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
                                           const HSweep<NT>& cur, const HSweep<NT>& nxt,
                                           int32_t ng, HSet<NT, Ones>& s0,
@@ -491,7 +498,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
     }
     int32_t g = 0;
     for (; g + 1 < ng; g += 2) {
-        h_load(cur, g + 1, s1, d);
+        h_load<NT, Ones, BL>(cur, g + 1, s1, d);
         __builtin_amdgcn_sched_barrier(0);
         a_permute(s0, paddr);
         h_mma(acc, accw, s0);
@@ -505,7 +512,8 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 #pragma unroll
         for (int t = 0; t < NT; ++t) w.xo[t] = more ? cur.xo[t] : nxt.xo[t];
         w.co = more ? cur.co : nxt.co;
-        h_load(w, more ? g + 2 : 0, s0, d);
+        if constexpr (BL) w.LB = more ? cur.LB : nxt.LB;
+        h_load<NT, Ones, BL>(w, more ? g + 2 : 0, s0, d);
         __builtin_amdgcn_sched_barrier(0);
         a_permute(s1, paddr);
         h_mma(acc, accw, s1);
@@ -515,11 +523,12 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
         a_permute(s0, paddr);
         h_mma(acc, accw, s0);
         __builtin_amdgcn_sched_barrier(0);
-        h_load(nxt, 0, s0, d);
+        h_load<NT, Ones, BL>(nxt, 0, s0, d);
     }
 }
 
 // The sweep `cur` of a super-tile, written to the ring (see run_sweep).
+template <bool BL>
 __device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSweep<4>& nxt,
                                                 int32_t ng, HSet<4>& s0,
                                                 int ring0, float* ring) {
@@ -527,7 +536,7 @@ __device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSwe
     const int kk = lane >> 4, col = lane & 15;
     mfma_v4 acc[4], unused[4];
     HSet<4> s1;
-    run_sweep<4, false>(acc, unused, cur, nxt, ng, s0, s1);
+    run_sweep<4, false, BL>(acc, unused, cur, nxt, ng, s0, s1);
     // rows 16 t + 4 kk .. + 3 of column col: one 16-byte store (ring0 is a multiple of 16
     // and kRingRows of 4, so the four rows never wrap apart)
 #pragma unroll
@@ -538,8 +547,16 @@ __device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSwe
     }
 }
 
-// The window rings of the four wavefronts of a workgroup (A sums, then B sums).
-__shared__ __attribute__((aligned(16))) float g_rings[2 * 4 * kRingFloats];
+// LDS of each wavefront of a workgroup in the diagonal-sum path (a quarter of 80 KiB, two
+// workgroups per CU): the ring of A sums, then either the staged template operands
+// (common weight) or the ring of B sums and the staged operands of both planes. Staged
+// operands: for each K group g of the width, the B operand of every lane l at 1024 g +
+// 16 l (S plane; the S^2 plane follows after ng groups).
+constexpr int kWaveLdsBytes = 20480;
+constexpr int kRingBytes = 4 * kRingFloats;
+constexpr int kStageGroupsConstW = (kWaveLdsBytes - kRingBytes) / 1024;        // 14
+constexpr int kStageGroupsWeighted = (kWaveLdsBytes - 2 * kRingBytes) / 2048;  // 4
+__shared__ __attribute__((aligned(16))) float g_wave_lds[4 * kWaveLdsBytes / 4];
 
 // Sample pair planes of one (period, width) item of the matrix-core paths: the A-sum
 // samples X (u with a common weight, v = u w otherwise) and the weights W (weighted
@@ -552,11 +569,14 @@ struct HPlanes {
 
 // Every candidate of one (period, width) item on the matrix cores, returning the lane's
 // first minimum of the statistic. C, C2: chunked template pair planes of S and S^2
-// (their zero blocks zrel bytes from C and C2). Reads of X stay below N + d + q + 16 (zero-filled past the gathered samples).
-// The window means: with TileMean (common weight, F <= 13), from the ones columns of the
-// same tiles, the window sum of u = sum_{c<F} G(k + c, 14) + G(k + F, 15) in float32;
-// otherwise from D, the float64 prefix sums of u.
-template <bool ConstW, bool TileMean>
+// (their zero blocks zrel bytes from C and C2). Reads of X stay below N + d + q + 16
+// (zero-filled past the gathered samples). The window means: with TileMean (common
+// weight, F <= 13), from the ones columns of the same tiles, the window sum of
+// u = sum_{c<F} G(k + c, 14) + G(k + F, 15) in float32; otherwise from D, the float64
+// prefix sums of u. With BL the template operands of every K group are staged once in the
+// wavefront's LDS (ng <= kStageGroupsConstW, or kStageGroupsWeighted for both planes)
+// instead of being read from memory by every super-tile.
+template <bool ConstW, bool TileMean, bool BL>
 __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __restrict__ C,
                                              const char* __restrict__ C2, int32_t zrel,
                                              const double* __restrict__ D,
@@ -564,8 +584,8 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     static_assert(ConstW || !TileMean, "tile means need the samples u");
     const int lane = lane_id();
     const int wave = uniform((int32_t)wave_id());
-    float* ringA = g_rings + wave * kRingFloats;
-    float* ringB = g_rings + (4 + wave) * kRingFloats;
+    float* ringA = g_wave_lds + wave * (kWaveLdsBytes / 4);
+    float* ringB = ringA + kRingFloats;
     const int32_t d = a.d, q = a.q;
     const int64_t K = a.K;
     const int32_t F = d / q;
@@ -575,11 +595,34 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     const WidthConstsF fc = width_consts_f(a);
     const float inv_xd = P.inv_x * fc.inv_d;
     const int32_t ng = chunk_pad(q) / kKGroup;
+    // the staged operands: after the ring(s); lane's slot of group 0 of each plane
+    lds_char* stage = (lds_char*)(ringA + (ConstW ? 1 : 2) * kRingFloats) + 16 * lane;
+    auto sweep = [&](const char* Xp, const char* Cp, int64_t st, bool ones, int plane) {
+        HSweep<4> w = h_sweep(Xp, Cp, zrel, q, F, rows, st, ones);
+        w.LB = stage + 1024 * ng * plane;
+        return w;
+    };
+    if constexpr (BL) {
+        // every lane's B operand of each K group (and plane), 4 groups of loads in flight
+        for (int pl = 0; pl < (ConstW ? 1 : 2); ++pl) {
+            const HSweep<4> w = sweep(P.X, pl ? C2 : C, 0, TileMean, pl);
+            const char* src = w.C + w.co;
+            for (int32_t g0 = 0; g0 < ng; g0 += 4) {
+                h8 v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = ld_h8(src + 64 * min(g0 + e, ng - 1));
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (g0 + e < ng)
+                        *(__attribute__((address_space(3))) h8*)(w.LB + 1024 * (g0 + e)) = v[e];
+            }
+        }
+    }
     int ring0 = 0;   // ring row of G row 64 T
     // sweeps in order (A plane, then B plane, of each super-tile); the first group of each
     // sweep is loaded during the one before (past the last super-tile: row block 0 again)
     HSet<4> ops;
-    h_load(h_sweep(P.X, C, zrel, q, F, rows, 0, TileMean), 0, ops);
+    h_load<4, false, BL>(sweep(P.X, C, 0, TileMean, 0), 0, ops);
     for (int64_t st = 0; st < nsuper; ++st) {
         const int64_t kc = 64 * st - F + lane;
         const bool cand = kc >= 0 && kc < K;
@@ -593,15 +636,13 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
             }
         }
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
-        const HSweep<4> wa = h_sweep(P.X, C, zrel, q, F, rows, st, TileMean);
+        const HSweep<4> wa = sweep(P.X, C, st, TileMean, 0);
         if constexpr (ConstW) {
-            super_tile_rows(wa, h_sweep(P.X, C, zrel, q, F, rows, stn, TileMean), ng, ops,
-                            ring0, ringA);
+            super_tile_rows<BL>(wa, sweep(P.X, C, stn, TileMean, 0), ng, ops, ring0, ringA);
         } else {
-            const HSweep<4> wb = h_sweep(P.W, C2, zrel, q, F, rows, st);
-            super_tile_rows(wa, wb, ng, ops, ring0, ringA);
-            super_tile_rows(wb, h_sweep(P.X, C, zrel, q, F, rows, stn), ng, ops, ring0,
-                            ringB);
+            const HSweep<4> wb = sweep(P.W, C2, st, false, 1);
+            super_tile_rows<BL>(wa, wb, ng, ops, ring0, ringA);
+            super_tile_rows<BL>(wb, sweep(P.X, C, stn, false, 0), ng, ops, ring0, ringB);
         }
         wave_sync_lds();
         // diagonal sums: all reads issue before the first add, columns past F masked
