@@ -106,8 +106,8 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     off = align_up(off + real * tmpl_padded);
     L.tmplsq = off;
     off = align_up(off + real * tmpl_padded);
-    // float32: float16 pair planes of the template rows (Toeplitz tiles, five planes) and of
-    // the chunked rows with their zero blocks (diagonal-sum tiles, four planes)
+    // float32: float16 pair planes of the template rows (Toeplitz tiles) and of the chunked
+    // rows with their zero blocks (diagonal-sum tiles), two planes each (S and S^2)
     if (mode == GTLS_MODE_FP32C) {
         L.tplen = tmpl_padded;
         L.n_chunked = 4 * b->total_tmpl + 1024 * b->total_widths + 1024;
@@ -157,7 +157,7 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     L.chunk = std::max<int64_t>(1, std::min<int64_t>(periods, ctx->chunk_bytes / per_period));
     L.n_chunks = (periods + L.chunk - 1) / L.chunk;
     L.counters = off;
-    off = align_up(off + 16 * 8 * L.n_chunks);
+    off = align_up(off + kChunkCounters * 8 * L.n_chunks);
     L.xbuf = off;
     off = align_up(off + 2 * real * L.xstride * L.chunk);
     L.dbuf = off;
@@ -241,7 +241,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
                        st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, ss);
     hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
                        b->y, b->dy, flat, wconst, (float2*)(ws + L.xsc));
-    const int64_t ncount = 16 * L.n_chunks;
+    const int64_t ncount = (int64_t)kChunkCounters * L.n_chunks;
     hipLaunchKernelGGL(zero_u64s_kernel, dim3((unsigned)std::min<int64_t>(1024, (ncount + tpb - 1) / tpb)),
                        dim3(tpb), 0, st, counters, ncount);
 
@@ -336,7 +336,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     for (int64_t ci = 0; ci < L.n_chunks && b->total_periods > 0; ++ci) {
         p.p0 = ci * L.chunk;
         p.chunk = std::min<int64_t>(L.chunk, b->total_periods - p.p0);
-        p.counters = counters + 16 * ci;
+        p.counters = counters + (int64_t)kChunkCounters * ci;
         const int64_t g1 = std::min<int64_t>(L.n_slots, p.chunk);
         hipError_t te = mark();
         hipLaunchKernelGGL(prep_kernel<Real>, dim3((unsigned)g1), dim3(kBlock), 0, st, p);

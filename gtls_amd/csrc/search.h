@@ -25,6 +25,14 @@ namespace gtls {
 constexpr int kBlock = 256;
 constexpr int64_t kTmplTail = 32768;  // least zeroed samples after the last template row
 constexpr int kScanPerThread = 11;  // gathered elements per thread per prefix-scan chunk
+// Queue counters sit 256 bytes apart (no two in one cache line: the atomics of different
+// XCDs on one line serialise); kChunkCounters per period chunk.
+constexpr int kCounterPitch = 32;
+constexpr int kChunkCounters = 16 * kCounterPitch;
+// Window items taken per queue atomic. With 1, consecutive items (the widths of one period)
+// go to different wavefronts, which then stream the period's samples through L1 and L2
+// together (measured: 2 items +0.6 %, 4 items +6 %, 8 items +18 % kernel time).
+constexpr int kItemBatch = 1;
 static_assert(kBlock * kScanPerThread >= kWinCap, "one prefix-scan chunk per sort window");
 
 
@@ -91,7 +99,8 @@ struct SearchParams {
     int64_t xstride;
     PeriodMeta* meta;
     ArgMin* part;                  // per (chunk period, width slot): the width's minimum
-    unsigned long long* counters;  // [0] prep queue, [1 .. 8] window queues (one per XCD)
+    unsigned long long* counters;  // queues, kCounterPitch apart: the prep queue, then the
+                                   // window queues (one per XCD)
     double* chi2;
     int32_t* best_k;
     int32_t* best_s;
@@ -562,10 +571,10 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
 }
 
 // Stage 3 of the chunk: persistent wavefronts, each taking (period, width slot) items
-// from the queue of its XCD -- the periods i = x, x + 8, ... of the chunk go to XCD x, so
-// the widths of one period are searched together and its samples stay in that XCD's L2 --
-// then from the other XCDs' queues once its own is drained. Every wavefront leaves after
-// it found all eight queues empty.
+// from the queue of its XCD, kItemBatch consecutive items per atomic -- the periods
+// i = x, x + 8, ... of the chunk go to XCD x, so the widths of one period are searched
+// together and its samples stay in that XCD's L2 -- then from the other XCDs' queues once
+// its own is drained. Every wavefront leaves after it found all eight queues empty.
 template <class Real>
 __global__ __launch_bounds__(kBlock, 2) void windows_kernel(SearchParams<Real> p) {
     const int lane = lane_id();
@@ -575,14 +584,13 @@ __global__ __launch_bounds__(kBlock, 2) void windows_kernel(SearchParams<Real> p
     int drained = 0;
     while (drained < 8) {
         unsigned long long it = 0;
-        if (lane == 0) it = atomicAdd(&p.counters[1 + x], 1ULL);
-        const int64_t item = uniform((int64_t)it);
+        if (lane == 0)
+            it = atomicAdd(&p.counters[(1 + x) * kCounterPitch], (unsigned long long)kItemBatch);
+        const int64_t item0 = uniform((int64_t)it);
         const int64_t nper = (p.chunk - x + 7) / 8;
         const int64_t nitems = nper > 0 ? nper * p.kmax : 0;
-        if (item >= nitems) {
-            ++drained;
-            x = (x + 1) & 7;
-        } else {
+        const int64_t iend = min(item0 + kItemBatch, nitems);
+        for (int64_t item = item0; item < iend; ++item) {
             const int64_t i = x + 8 * (item / p.kmax);
             const int32_t j = (int32_t)(item % p.kmax);
             PeriodMeta pm = p.meta[i];
@@ -597,6 +605,10 @@ __global__ __launch_bounds__(kBlock, 2) void windows_kernel(SearchParams<Real> p
                 best = wave_argmin(best);
                 if (lane == 0) p.part[i * p.kmax + j] = best;
             }
+        }
+        if (item0 >= nitems) {
+            ++drained;
+            x = (x + 1) & 7;
         }
     }
 }

@@ -527,6 +527,158 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Diagonal-sum sweeps over K-group pairs. The sample rows of the diagonal-sum tiles start
+// at arbitrary 4 q-byte offsets; one load per K group (16 rows x 64 bytes) fetches most
+// 128-byte cache lines twice, by two loads in flight together. Here a load covers 8 rows x
+// 128 bytes (two K groups) instead: lane l reads row 8 h + (l / 4) % 8 of the tile (h = the
+// tile's load 0 or 1), K group 2 p + l / 32, slice l % 4 (16 bytes). v_permlane32_swap then
+// gathers the first K group of all 16 rows into the load-0 register and the second into
+// the load-1 register (lanes 32 h + 4 r8 + s: row 8 h + r8, slice s), and the same
+// ds_bpermute as a_permute (source lane 4 r + s) moves them to the operand layout.
+// ---------------------------------------------------------------------------------------
+struct PSweep {
+    const char* X;       // sample pair plane (wave-uniform)
+    const char* C;       // template pair plane (wave-uniform)
+    int32_t rb;          // lane's row in tile 0, load 0: 64 st + (l / 4) % 8
+    int32_t co;          // lane's B operand byte offset (K group 0)
+    lds_char* LB;        // lane's staged B operand of group 0 (BL)
+};
+
+// Super-tile st's pair sweep over X and C (B operands and row rules as h_sweep).
+__device__ __forceinline__ PSweep p_sweep(const char* __restrict__ X, const char* __restrict__ C,
+                                          int32_t zrel, int32_t q, int32_t F, int64_t st,
+                                          bool ones, lds_char* LB) {
+    const int lane = lane_id();
+    const int kk = lane >> 4, col = lane & 15;
+    PSweep w;
+    w.X = X;
+    w.C = C;
+    w.rb = (int32_t)(64 * st) + ((lane >> 2) & 7);
+    const int32_t ch = col <= F ? col : (ones && col >= kOnesCol ? F + 1 + (col - kOnesCol) : -1);
+    w.co = (ch >= 0) ? 4 * (ch * chunk_pad(q) + 4 * kk) : zrel + 16 * kk;
+    w.LB = LB;
+    return w;
+}
+
+// One K-group pair's operands: per tile the two loads, and the B operands of both groups.
+struct PSet {
+    h8 a[4][2];
+    h8 b[2];
+};
+
+// Loads pair p (< ng_pairs) of a sweep (rows q samples apart, rows past `rows` read
+// block 0); the B operand of a second group past ng - 1 (odd ng) repeats group ng - 1 and
+// is not used.
+template <bool BL>
+__device__ __forceinline__ void p_load(const PSweep& w, int32_t p, int32_t ng, int32_t q,
+                                       int32_t rows, PSet& o) {
+    const int lane = lane_id();
+    const uint32_t off = 128u * (uint32_t)p + 64u * (uint32_t)(lane >> 5) + 16u * (uint32_t)(lane & 3);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int32_t row = w.rb + 16 * t + 8 * h;
+            const uint32_t ro = row < rows ? 4u * (uint32_t)row * (uint32_t)q : 0u;
+            o.a[t][h] = ld_h8(w.X + (ro + off));
+        }
+    const int32_t g0 = 2 * p, g1 = min(2 * p + 1, ng - 1);
+    if constexpr (BL) {
+        o.b[0] = *(const __attribute__((address_space(3))) h8*)(w.LB + 1024 * g0);
+        o.b[1] = *(const __attribute__((address_space(3))) h8*)(w.LB + 1024 * g1);
+    } else {
+        o.b[0] = ld_h8(w.C + (int64_t)(w.co + 64 * g0));
+        o.b[1] = ld_h8(w.C + (int64_t)(w.co + 64 * g1));
+    }
+}
+
+// The operand layout of both K groups (permlane32 swap, then ds_bpermute), then their
+// MFMAs (the second group only when `two`).
+__device__ __forceinline__ void p_mma(mfma_v4 (&acc)[4], PSet& o, int paddr, bool two) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        uint32_t v0[4], v1[4];
+        __builtin_memcpy(v0, &o.a[t][0], sizeof(v0));
+        __builtin_memcpy(v1, &o.a[t][1], sizeof(v1));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(v0[e], v1[e], false, false);
+            v0[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(paddr, (int)sw[0]);
+            v1[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(paddr, (int)sw[1]);
+        }
+        __builtin_memcpy(&o.a[t][0], v0, sizeof(v0));
+        __builtin_memcpy(&o.a[t][1], v1, sizeof(v1));
+    }
+    HSet<4> g;
+    g.b = o.b[0];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) g.a[t] = o.a[t][0];
+    mfma_v4 unused[4];
+    h_mma(acc, unused, g);
+    if (two) {
+        g.b = o.b[1];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) g.a[t] = o.a[t][1];
+        h_mma(acc, unused, g);
+    }
+}
+
+// One pair sweep into acc (zeroed here), pipelined as run_sweep over pairs: `cur` with its
+// pair 0 already in s0; on return s0 holds pair 0 of `nxt`.
+template <bool BL>
+__device__ __forceinline__ void run_psweep(mfma_v4 (&acc)[4], const PSweep& cur,
+                                           const PSweep& nxt, int32_t ng, int32_t q,
+                                           int32_t rows, PSet& s0, PSet& s1) {
+    const int paddr = a_permute_addr();
+    const int32_t np = (ng + 1) >> 1;
+    const bool odd = (ng & 1) != 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
+    int32_t p = 0;
+    for (; p + 1 < np; p += 2) {
+        p_load<BL>(cur, p + 1, ng, q, rows, s1);
+        __builtin_amdgcn_sched_barrier(0);
+        p_mma(acc, s0, paddr, true);
+        __builtin_amdgcn_sched_barrier(0);
+        // pair p + 2 of this sweep, or pair 0 of the next one
+        const bool more = p + 2 < np;
+        PSweep w;
+        w.X = more ? cur.X : nxt.X;
+        w.C = more ? cur.C : nxt.C;
+        w.rb = more ? cur.rb : nxt.rb;
+        w.co = more ? cur.co : nxt.co;
+        w.LB = more ? cur.LB : nxt.LB;
+        p_load<BL>(w, more ? p + 2 : 0, ng, q, rows, s0);
+        __builtin_amdgcn_sched_barrier(0);
+        p_mma(acc, s1, paddr, !(odd && p + 2 == np));
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (p < np) {   // an odd pair count: the last pair is in s0
+        p_mma(acc, s0, paddr, !odd);
+        __builtin_amdgcn_sched_barrier(0);
+        p_load<BL>(nxt, 0, ng, q, rows, s0);
+    }
+}
+
+// The pair sweep `cur` of a super-tile, written to the ring (see run_psweep).
+template <bool BL>
+__device__ __forceinline__ void super_tile_prows(const PSweep& cur, const PSweep& nxt, int32_t ng,
+                                                 int32_t q, int32_t rows, PSet& s0, int ring0,
+                                                 float* ring) {
+    const int lane = lane_id();
+    const int kk = lane >> 4, col = lane & 15;
+    mfma_v4 acc[4];
+    PSet s1;
+    run_psweep<BL>(acc, cur, nxt, ng, q, rows, s0, s1);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        int r = ring0 + 16 * t + 4 * kk;
+        r = r >= kRingRows ? r - kRingRows : r;
+        *(mfma_v4*)(ring + col * kRingPitch + r) = acc[t];
+    }
+}
+
 // The sweep `cur` of a super-tile, written to the ring (see run_sweep).
 template <bool BL>
 __device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSweep<4>& nxt,
@@ -598,14 +750,12 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     // the staged operands: after the ring(s); lane's slot of group 0 of each plane
     lds_char* stage = (lds_char*)(ringA + (ConstW ? 1 : 2) * kRingFloats) + 16 * lane;
     auto sweep = [&](const char* Xp, const char* Cp, int64_t st, bool ones, int plane) {
-        HSweep<4> w = h_sweep(Xp, Cp, zrel, q, F, rows, st, ones);
-        w.LB = stage + 1024 * ng * plane;
-        return w;
+        return p_sweep(Xp, Cp, zrel, q, F, st, ones, stage + 1024 * ng * plane);
     };
     if constexpr (BL) {
         // every lane's B operand of each K group (and plane), 4 groups of loads in flight
         for (int pl = 0; pl < (ConstW ? 1 : 2); ++pl) {
-            const HSweep<4> w = sweep(P.X, pl ? C2 : C, 0, TileMean, pl);
+            const PSweep w = sweep(P.X, pl ? C2 : C, 0, TileMean, pl);
             const char* src = w.C + w.co;
             for (int32_t g0 = 0; g0 < ng; g0 += 4) {
                 h8 v[4];
@@ -621,8 +771,9 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     int ring0 = 0;   // ring row of G row 64 T
     // sweeps in order (A plane, then B plane, of each super-tile); the first group of each
     // sweep is loaded during the one before (past the last super-tile: row block 0 again)
-    HSet<4> ops;
-    h_load<4, false, BL>(sweep(P.X, C, 0, TileMean, 0), 0, ops);
+    PSet ops;
+    const int32_t rows32 = (int32_t)rows;
+    p_load<BL>(sweep(P.X, C, 0, TileMean, 0), 0, ng, q, rows32, ops);
     for (int64_t st = 0; st < nsuper; ++st) {
         const int64_t kc = 64 * st - F + lane;
         const bool cand = kc >= 0 && kc < K;
@@ -636,13 +787,15 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
             }
         }
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
-        const HSweep<4> wa = sweep(P.X, C, st, TileMean, 0);
+        const PSweep wa = sweep(P.X, C, st, TileMean, 0);
         if constexpr (ConstW) {
-            super_tile_rows<BL>(wa, sweep(P.X, C, stn, TileMean, 0), ng, ops, ring0, ringA);
+            super_tile_prows<BL>(wa, sweep(P.X, C, stn, TileMean, 0), ng, q, rows32, ops, ring0,
+                                 ringA);
         } else {
-            const HSweep<4> wb = sweep(P.W, C2, st, false, 1);
-            super_tile_rows<BL>(wa, wb, ng, ops, ring0, ringA);
-            super_tile_rows<BL>(wb, sweep(P.X, C, stn, false, 0), ng, ops, ring0, ringB);
+            const PSweep wb = sweep(P.W, C2, st, false, 1);
+            super_tile_prows<BL>(wa, wb, ng, q, rows32, ops, ring0, ringA);
+            super_tile_prows<BL>(wb, sweep(P.X, C, stn, false, 0), ng, q, rows32, ops, ring0,
+                                 ringB);
         }
         wave_sync_lds();
         // diagonal sums: all reads issue before the first add, columns past F masked
