@@ -399,6 +399,12 @@ __device__ __forceinline__ void wg_fold_sort(const double* __restrict__ t, int64
     __syncthreads();
 }
 
+// The windowed form keeps bucket b's count / position at hist[hslot(b)]: one spare int per
+// 32 buckets, so that threads scanning consecutive 32-bucket runs start in different LDS
+// banks (33 is odd). Its histogram array holds kHistLen ints.
+constexpr int kHistLen = kMaxBuckets + kMaxBuckets / 32;
+__device__ __forceinline__ int hslot(int b) { return b + (b >> 5); }
+
 constexpr int kWinCap = 2816;                        // samples one window stages in LDS
 constexpr int kWinTarget = kWinCap - kWaveSortMax;   // window size before its last bucket
 constexpr int kWinMid = kWinCap / 17 + 1;            // buckets of 17+ keys in one window
@@ -406,7 +412,7 @@ constexpr int kWinPer = (kWinCap + 255) / 256;       // staged samples per threa
 constexpr int kMaxWin = 255;                         // windows of one period (uint8 table)
 
 // Windowed stable phase sort of the n samples of t for period P (see the file comment).
-// Scratch: binned (n ints: sample indices grouped by window). LDS: hist (kMaxBuckets ints), red (blockDim/64 ints), counts (4 ints), sphi /
+// Scratch: binned (n ints: sample indices grouped by window). LDS: hist (kHistLen ints), red (blockDim/64 ints), counts (4 ints), sphi /
 // sidx (kWinCap doubles / ints), mids (kWinMid int64), wtab (kMaxBuckets bytes: window of
 // each bucket), wb / ws (kMaxWin + 1 ints: first bucket / first position of each window),
 // wcur (kMaxWin ints).
@@ -423,7 +429,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     int32_t* sidx, int64_t* mids, uint8_t* wtab, int* wb, int* ws, int* wcur, Sink sink) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int B = buckets_for(n);
-    for (int b = tid; b < B; b += nt) hist[b] = 0;
+    for (int b = tid; b < B; b += nt) hist[hslot(b)] = 0;
     if (tid < 4) counts[tid] = 0;
     __syncthreads();
     constexpr int U = 8;   // samples in flight per thread
@@ -438,7 +444,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         for (int u = 0; u < U; ++u) {
             const int64_t j = base + (int64_t)u * nt + tid;
             if (j < n) {
-                atomicAdd(&hist[phase_bucket(fold_phase(tv[u], P), B, neg)], 1);
+                atomicAdd(&hist[hslot(phase_bucket(fold_phase(tv[u], P), B, neg))], 1);
             }
         }
     }
@@ -449,15 +455,15 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         const int base = tid * per;
         int sum = 0, mx = 0;
         for (int e = 0; e < per; ++e) {
-            const int v = hist[base + e];
+            const int v = hist[hslot(base + e)];
             sum += v;
             mx = max(mx, v);
         }
         int total;
         int run = block_exclusive_scan(sum, red, total);
         for (int e = 0; e < per; ++e) {
-            const int v = hist[base + e];
-            hist[base + e] = run;
+            const int v = hist[hslot(base + e)];
+            hist[hslot(base + e)] = run;
             run += v;
         }
         if (mx > kWaveSortMax) atomicOr(&counts[0], 1);
@@ -471,26 +477,48 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         if (too_big) return false;
     }
     // the windows: buckets wb[w] .. wb[w + 1] - 1, wb[w + 1] the first bucket starting at or
-    // past ws[w] + kWinTarget, so that a window holds fewer than kWinCap samples
-    if (tid == 0) {
+    // past ws[w] + kWinTarget, so that a window holds fewer than kWinCap samples. The first
+    // wavefront finds each boundary by 64-way probes of the bucket starts (monotone): three
+    // rounds for 8192 buckets instead of a 13-step chain of dependent reads.
+    if (uniform((int32_t)wave_id()) == 0) {
+        const int lane = lane_id();
         int K = 0, bfirst = 0, wstart = 0;
         while (bfirst < B && K < kMaxWin) {
-            int lo = bfirst + 1, hi = B;
+            // first b in [bfirst + 1, B) with hist[b] >= target, else B; the answer stays
+            // in [lo, hi] with hi == B or hist[hi] >= target
             const int target = wstart + kWinTarget;
+            int lo = bfirst + 1, hi = B;
             while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (hist[mid] >= target) hi = mid; else lo = mid + 1;
+                const int step = (hi - lo + kWave - 1) / kWave;
+                const int b = lo + lane * step;
+                const bool ge = b >= hi || hist[hslot(min(b, B - 1))] >= target;
+                const unsigned long long m = __ballot(ge);
+                if (m == 0ULL) {
+                    lo = lo + (kWave - 1) * step + 1;   // every probe below the target
+                } else {
+                    const int f = __ffsll((long long)m) - 1;
+                    if (f == 0) {
+                        hi = lo;
+                    } else {
+                        hi = min(hi, lo + f * step);
+                        lo = lo + (f - 1) * step + 1;
+                    }
+                }
             }
-            wb[K] = bfirst;
-            ws[K] = wstart;
+            if (lane == 0) {
+                wb[K] = bfirst;
+                ws[K] = wstart;
+            }
             ++K;
             bfirst = lo;
-            wstart = lo < B ? hist[lo] : (int)n;
+            wstart = lo < B ? hist[hslot(lo)] : (int)n;
         }
-        wb[K] = B;
-        ws[K] = (int)n;
-        counts[2] = K;
-        counts[3] = bfirst < B ? 1 : 0;   // more windows needed than the table holds
+        if (lane == 0) {
+            wb[K] = B;
+            ws[K] = (int)n;
+            counts[2] = K;
+            counts[3] = bfirst < B ? 1 : 0;   // more windows needed than the table holds
+        }
     }
     __syncthreads();
     const int K = counts[2];
@@ -499,13 +527,21 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         __syncthreads();   // as above: all reads of counts[] done before a return
         if (too_many) return false;
     }
-    for (int b = tid; b < B; b += nt) {
-        int lo = 0, hi = K;   // last w with wb[w] <= b
+    // window of every bucket: each thread takes B / nt consecutive buckets, finds the window
+    // of the first one (last w with wb[w] <= b) and walks on (wb[K] = B ends the walk)
+    {
+        const int per = B / nt;
+        const int b0 = tid * per;
+        int lo = 0, hi = K;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (wb[mid] <= b) lo = mid; else hi = mid;
+            if (wb[mid] <= b0) lo = mid; else hi = mid;
         }
-        wtab[b] = (uint8_t)lo;
+        int next = wb[lo + 1];
+        for (int b = b0; b < b0 + per; ++b) {
+            while (b >= next) next = wb[++lo + 1];
+            wtab[b] = (uint8_t)lo;
+        }
     }
     for (int w = tid; w < K; w += nt) wcur[w] = ws[w];
     __syncthreads();
@@ -556,7 +592,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
             for (int u = 0; u < kWinPer; ++u) {
                 if (jj[u] >= 0) {
                     const double ph = fold_phase(tv[u], P);
-                    const int pos = atomicAdd(&hist[phase_bucket(ph, B, neg)], 1) - wstart;
+                    const int pos = atomicAdd(&hist[hslot(phase_bucket(ph, B, neg))], 1) - wstart;
                     sphi[pos] = ph;
                     sidx[pos] = (int32_t)jj[u];
                 }
@@ -565,8 +601,8 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         __syncthreads();
         // hist[b] now holds the end of bucket b for the window's buckets
         for (int b = bfirst + tid; b < bnext; b += nt) {
-            const int blo = (b > bfirst ? hist[b - 1] : wstart) - wstart;
-            const int cnt = hist[b] - wstart - blo;
+            const int blo = (b > bfirst ? hist[hslot(b - 1)] : wstart) - wstart;
+            const int cnt = hist[hslot(b)] - wstart - blo;
             // one network per pass of the wavefront: the 16-key one for all of its lanes
             // when any of them holds 9-16 keys (lanes would otherwise run both networks)
             const bool wide = __ballot(cnt > 8 && cnt <= 16) != 0;

@@ -287,13 +287,22 @@ __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw
 }
 
 // The samples of one sort window in float32 with a common weight when no running sums are
-// wanted: position o0 + j takes the (hi, lo) pair of sx u of sample sidx[j]. All threads
-// call it; no barrier.
+// wanted: position o0 + j takes the (hi, lo) pair of sx u of sample sidx[j]. All of a
+// thread's gathers are in flight before its first store. All threads call it; no barrier.
 __device__ __forceinline__ void gather_window_pairs(const float2* __restrict__ uw, int64_t o0,
                                                     int cnt, const int32_t* sidx, uint32_t* hb,
                                                     float sx) {
-    for (int j = threadIdx.x; j < cnt; j += kBlock)
-        __builtin_nontemporal_store(split_pair(uw[sidx[j]].x * sx), &hb[o0 + j]);
+    float u[kWinPer];
+#pragma unroll
+    for (int e = 0; e < kWinPer; ++e) {
+        const int j = threadIdx.x + e * kBlock;
+        u[e] = j < cnt ? uw[sidx[j]].x : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < kWinPer; ++e) {
+        const int j = threadIdx.x + e * kBlock;
+        if (j < cnt) __builtin_nontemporal_store(split_pair(u[e] * sx), &hb[o0 + j]);
+    }
 }
 
 // Stage 1 + 2 of each period of the chunk, one workgroup per period (persistent):
@@ -303,7 +312,7 @@ __device__ __forceinline__ void gather_window_pairs(const float2* __restrict__ u
 template <class Real>
 __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
     using R2 = typename Pair<Real>::type;
-    __shared__ __attribute__((aligned(16))) int hist[kMaxBuckets];
+    __shared__ __attribute__((aligned(16))) int hist[kHistLen];
     __shared__ __attribute__((aligned(16))) double s_phi[kWinCap];
     __shared__ int32_t s_idx[kWinCap];
     __shared__ int64_t s_mids[kWinMid];
@@ -651,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void fold_sort_kernel(const double* t, int6
                                                           const double* periods,
                                                           int64_t n_periods, int neg,
                                                           char* slots, int32_t* perm_out) {
-    __shared__ __attribute__((aligned(16))) int hist[kMaxBuckets];
+    __shared__ __attribute__((aligned(16))) int hist[kHistLen];
     __shared__ __attribute__((aligned(16))) double s_phi[kWinCap];
     __shared__ int32_t s_idx[kWinCap];
     __shared__ int64_t s_mids[kWinMid];
