@@ -4,14 +4,14 @@ The tunables are those of gtls_set_tuning: the stride below which a width takes 
 Toeplitz matrix-core tiles instead of the diagonal-sum tiles, the buffer budget of one
 period chunk (which sets how many kernel tails a call pays), and the resident workgroups
 per CU of the persistent search kernels (waves per SIMD: one or two). The K group of the
-tiles is fixed at 16 samples by the matrix instruction (16x16x32 over sample pairs). `autotune` times one search
-call of a sample batch (curves spread over the batch's sizes) under each candidate setting
-with the library's per-kernel HIP events, and applies the fastest to the context of the
-device's current stream; later searches from this process on that stream use it.
+tiles is fixed at 16 samples by the matrix instruction (16x16x32 over sample pairs).
+`autotune` times search calls of a sample batch (curves spread over the batch's sizes) with
+the library's per-kernel HIP events, one axis at a time (workgroups per CU, then the chunk
+budget, then the Toeplitz bound, each axis over all its values with the others at their
+best so far), and applies the fastest setting to the context of the device's current
+stream; later searches from this process on that stream use it.
 """
 from __future__ import annotations
-
-import itertools
 
 import numpy as np
 
@@ -42,21 +42,19 @@ def sample_batch(db, n=32):
     return DeviceBatch(BatchPlan([bp.plans[i] for i in sorted(set(pick.tolist()))]), db.dev.index)
 
 
-def autotune(db, skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, n_sample=32, repeats=2):
-    """Times the sample's search under every (toep_q, chunk_mb, wg_per_cu) candidate (the
-    best of `repeats` calls each, after one untimed call) and applies the fastest. Returns
-    (best setting, {setting: milliseconds})."""
+def autotune(db, skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, n_sample=32, repeats=1):
+    """Coordinate search over (toep_q, chunk_mb, wg_per_cu): after one untimed call, each
+    axis in turn (WG_PER_CU, CHUNK_MB, TOEP_Q) is timed over all of its values with the other
+    two at their best so far (the best of `repeats` calls per setting), and the fastest
+    setting seen is applied. Returns (best setting, {setting: milliseconds})."""
     import torch
     sb = sample_batch(db, n_sample)
     dev = db.dev.index
     times = {}
-    set_kernel_timing(dev, True)
-    try:
-        for setting in itertools.product(TOEP_Q, CHUNK_MB, WG_PER_CU):
+
+    def timed(setting):
+        if setting not in times:
             set_tuning(dev, *setting)
-            sb.search("fp32c", skip_point, depth_min)
-            torch.cuda.synchronize(db.dev)
-            kernel_ms(dev)
             best = np.inf
             for _ in range(repeats):
                 sb.search("fp32c", skip_point, depth_min)
@@ -64,6 +62,18 @@ def autotune(db, skip_point=8, depth_min=C.TRANSIT_DEPTH_MIN, n_sample=32, repea
                 (pm, wm, rm), _ = kernel_ms(dev)
                 best = min(best, pm + wm + rm)
             times[setting] = best
+        return times[setting]
+
+    cur = (TOEP_Q[0], CHUNK_MB[-1], WG_PER_CU[-1])
+    set_kernel_timing(dev, True)
+    try:
+        set_tuning(dev, *cur)
+        sb.search("fp32c", skip_point, depth_min)   # kernels loaded, workspace allocated
+        torch.cuda.synchronize(db.dev)
+        kernel_ms(dev)
+        for axis, values in ((2, WG_PER_CU), (1, CHUNK_MB), (0, TOEP_Q)):
+            cands = [cur[:axis] + (v,) + cur[axis + 1:] for v in values]
+            cur = min(cands, key=timed)
     finally:
         set_kernel_timing(dev, False)
     choice = min(times, key=times.get)
