@@ -28,7 +28,7 @@ EXPORTED = (
     "gtls_spectra_workspace_bytes", "gtls_spectra_batched", "gtls_spectra_batched_noalloc",
     "gtls_transit_fwd",
     "gtls_transit_bwd", "gtls_transit_chi2", "gtls_set_timing", "gtls_kernel_ms", "gtls_set_tuning",
-    "gtls_set_graph_timing",
+    "gtls_set_graph_timing", "gtls_transit_tile_points", "gtls_transit_chi2_grouped",
 )
 
 P = ct.c_void_p
@@ -84,6 +84,9 @@ def _declare(lib):
         "gtls_kernel_ms": (I64, [P, ct.POINTER(F64)]),
         "gtls_set_tuning": (ct.c_int, [P, I32, I64, I32]),
         "gtls_set_graph_timing": (ct.c_int, [P, P, I64]),
+        "gtls_transit_tile_points": (I64, []),
+        "gtls_transit_chi2_grouped": (ct.c_int, [P, I32, I32, I64, I32, P, P, P, P, I64, P, P, P,
+                                                 P, I64, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -146,3 +149,12 @@ class Context:
 
     def slots(self):
         return load().gtls_search_slots(self.handle)
+
+    @classmethod
+    def close_all(cls):
+        """Destroys every cached context (gtls_destroy); the caller has finished all work on
+        their streams. Later calls create new ones."""
+        lib = load()
+        for ctx in cls._cache.values():
+            lib.gtls_destroy(ctx.handle)
+        cls._cache.clear()

@@ -807,4 +807,62 @@ int gtls_transit_chi2(gtls_ctx* ctx, int32_t mode, int64_t n_models, const doubl
     return GTLS_OK;
 }
 
+int64_t gtls_transit_tile_points(void) { return kChi2Tile; }
+
+int gtls_transit_chi2_grouped(gtls_ctx* ctx, int32_t mode, int32_t n_curves, int64_t n_models,
+                              int32_t max_models, const double* theta, const int64_t* model_off,
+                              const int64_t* pt_off, const int64_t* tile_off, int64_t n_tiles,
+                              const double* t, const double* y, const double* w, double* part,
+                              int64_t part_len, double* chi2, double* grad_theta) {
+    if (!ctx) return GTLS_E_SHAPE;
+    if (mode != GTLS_MODE_FP64 && mode != GTLS_MODE_FP32C)
+        return fail(ctx, GTLS_E_MODE, "unknown mode");
+    if (n_curves <= 0 || n_models < 0 || n_models > (1LL << 31) - 1 || n_tiles < 0 ||
+        n_tiles > (1LL << 31) - 1 || max_models < 0)
+        return fail(ctx, GTLS_E_SHAPE, "bad curve, model or tile count");
+    if (n_models == 0) return GTLS_OK;
+    if (max_models < 1) return fail(ctx, GTLS_E_SHAPE, "max_models must be positive");
+    if (!theta || !model_off || !pt_off || !tile_off || !t || !y || !w || !chi2)
+        return fail(ctx, GTLS_E_SHAPE, "null transit model argument");
+    const int sums = grad_theta ? 7 : 1;
+    if (n_tiles > 0 && (!part || part_len < n_tiles * (int64_t)max_models * sums))
+        return fail(ctx, GTLS_E_WORKSPACE, "tile sums need n_tiles * max_models * (7 or 1) doubles");
+    DeviceGuard guard(ctx->device);
+    const dim3 tiles((unsigned)n_tiles), block(kChi2Threads);
+    if (n_tiles > 0) {
+        if (mode == GTLS_MODE_FP64) {
+            if (grad_theta)
+                hipLaunchKernelGGL((transit_chi2_tile_kernel<double, true>), tiles, block, 0,
+                                   ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
+                                   max_models, t, y, w, part);
+            else
+                hipLaunchKernelGGL((transit_chi2_tile_kernel<double, false>), tiles, block, 0,
+                                   ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
+                                   max_models, t, y, w, part);
+        } else {
+            if (grad_theta)
+                hipLaunchKernelGGL((transit_chi2_tile_kernel<float, true>), tiles, block, 0,
+                                   ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
+                                   max_models, t, y, w, part);
+            else
+                hipLaunchKernelGGL((transit_chi2_tile_kernel<float, false>), tiles, block, 0,
+                                   ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
+                                   max_models, t, y, w, part);
+        }
+    }
+    const int64_t nsum = n_models * sums;
+    const dim3 sgrid((unsigned)((nsum + 255) / 256)), sblock(256);
+    if (grad_theta)
+        hipLaunchKernelGGL(transit_chi2_sum_kernel<true>, sgrid, sblock, 0, ctx->stream,
+                           model_off, tile_off, n_curves, max_models, n_models, part, chi2,
+                           grad_theta);
+    else
+        hipLaunchKernelGGL(transit_chi2_sum_kernel<false>, sgrid, sblock, 0, ctx->stream,
+                           model_off, tile_off, n_curves, max_models, n_models, part, chi2,
+                           grad_theta);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "grouped transit chi2 launch");
+    return GTLS_OK;
+}
+
 }  // extern "C"

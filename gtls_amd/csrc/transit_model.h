@@ -239,4 +239,119 @@ __global__ __launch_bounds__(chi2_block<Real>()) void transit_chi2_kernel(const 
     }
 }
 
+// The same loss and gradient with the models grouped by curve, in one pass over each
+// curve's points. The points of curve c (pt_off) are cut into tiles of kChi2Tile
+// consecutive points, tiles [tile_off[c], tile_off[c + 1]) of the launch; the models of
+// curve c are [model_off[c], model_off[c + 1]) (at most max_models). Workgroup i takes tile
+// i: its kChi2PerThread points per thread stay in registers while every model of the curve
+// is evaluated on them, and the tile's sums go to
+//     part[(i * max_models + j) * S + k]      (model m = model_off[c] + j, k < S),
+// S = 7 with Grad (chi2, then the six gradient components), else 1. Each point is read
+// once per call, whatever the number of models. transit_chi2_sum_kernel then adds every
+// model's tiles in tile order, so the result does not depend on the launch.
+constexpr int kChi2Threads = 256;
+constexpr int kChi2PerThread = 8;
+constexpr int kChi2Tile = kChi2Threads * kChi2PerThread;
+
+// Last index c in [0, n) with off[c] <= i, for off ascending with off[0] <= i < off[n].
+__device__ __forceinline__ int owner_of(const int64_t* off, int n, int64_t i) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <class Real, bool Grad>
+__global__ __launch_bounds__(kChi2Threads) void transit_chi2_tile_kernel(
+    const double* __restrict__ theta, const int64_t* __restrict__ model_off,
+    const int64_t* __restrict__ pt_off, const int64_t* __restrict__ tile_off, int32_t n_curves,
+    int32_t max_models, const double* __restrict__ t, const double* __restrict__ y,
+    const double* __restrict__ w, double* __restrict__ part) {
+    constexpr int kSums = Grad ? 7 : 1;
+    __shared__ double red[2][kSums][kChi2Threads / kWave];
+    const int64_t i = blockIdx.x;
+    const int c = owner_of(tile_off, n_curves, i);
+    const int64_t a = pt_off[c] + (i - tile_off[c]) * kChi2Tile;
+    const int64_t cnt = min((int64_t)kChi2Tile, pt_off[c + 1] - a);
+    const int tid = threadIdx.x;
+    // the tile's points (slots past the curve: weight 0, so they add nothing)
+    double tv[kChi2PerThread], yv[kChi2PerThread], wv[kChi2PerThread];
+#pragma unroll
+    for (int u = 0; u < kChi2PerThread; ++u) {
+        const int64_t j = (int64_t)u * kChi2Threads + tid;
+        const bool in = j < cnt;
+        const int64_t g = a + (in ? j : 0);
+        tv[u] = in ? t[g] : 0.0;
+        yv[u] = in ? y[g] : 1.0;
+        wv[u] = in ? w[g] : 0.0;
+    }
+    const int64_t m0 = model_off[c], m1 = model_off[c + 1];
+    const int lane = tid & (kWave - 1), wave = tid / kWave;
+    for (int64_t m = m0; m < m1; ++m) {
+        double th[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) th[k] = theta[6 * m + k];
+        const double inv_p = 1.0 / th[0];
+        double acc[kSums];
+#pragma unroll
+        for (int k = 0; k < kSums; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int u = 0; u < kChi2PerThread; ++u) {
+            const TransitPoint<Real> pt = transit_point<Real, Grad>(tv[u], th, inv_p);
+            const double r = yv[u] - (double)pt.f;
+            const double wr = wv[u] * r;
+            acc[0] = fma(wr, r, acc[0]);
+            if constexpr (Grad) {
+                const double g = -2.0 * wr;
+                acc[1] = fma(g, (double)pt.dP, acc[1]);
+                acc[2] = fma(g, (double)pt.dT0, acc[2]);
+                acc[3] = fma(g, (double)pt.dD, acc[3]);
+                acc[4] = fma(g, (double)pt.ddelta, acc[4]);
+                acc[5] = fma(g, (double)pt.dq1, acc[5]);
+                acc[6] = fma(g, (double)pt.dq2, acc[6]);
+            }
+        }
+        // workgroup sums; the two halves of red alternate between models, so one barrier
+        // per model orders every write of red before its reads and after the reads of the
+        // model two back
+        const int h = (int)((m - m0) & 1);
+#pragma unroll
+        for (int k = 0; k < kSums; ++k) {
+            const double v = wave_sum(acc[k]);
+            if (lane == 0) red[h][k][wave] = v;
+        }
+        __syncthreads();
+        if (tid < kSums) {
+            double sum = 0.0;
+#pragma unroll
+            for (int v = 0; v < kChi2Threads / kWave; ++v) sum += red[h][tid][v];
+            part[(i * max_models + (m - m0)) * kSums + tid] = sum;
+        }
+    }
+}
+
+// chi2[m] (and grad_theta[6 m + k]) as the sum of model m's tile sums in tile order; one
+// thread per (model, sum).
+template <bool Grad>
+__global__ void transit_chi2_sum_kernel(const int64_t* __restrict__ model_off,
+                                        const int64_t* __restrict__ tile_off, int32_t n_curves,
+                                        int32_t max_models, int64_t n_models,
+                                        const double* __restrict__ part, double* __restrict__ chi2,
+                                        double* __restrict__ grad_theta) {
+    constexpr int kSums = Grad ? 7 : 1;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_models * kSums) return;
+    const int64_t m = idx / kSums;
+    const int k = (int)(idx - m * kSums);
+    const int c = owner_of(model_off, n_curves, m);
+    const int64_t j = m - model_off[c];
+    double sum = 0.0;
+    for (int64_t i = tile_off[c]; i < tile_off[c + 1]; ++i)
+        sum += part[(i * max_models + j) * kSums + k];
+    if (k == 0) chi2[m] = sum;
+    else grad_theta[6 * m + k - 1] = sum;
+}
+
 }  // namespace gtls

@@ -3,7 +3,8 @@
 Each grid peak of a light curve (period, best width and T0 position of the search) seeds a
 model with parameters theta = (P, T0, duration, depth, q1, q2); the models are fitted to
 their curves by Adam on the weighted chi2 sum_i w_i (y_i - f_i)^2. The per-model chi2 and
-its gradient come from one fused HIP kernel pass (gtls_transit_chi2, wrapped as the
+its gradient come from one fused HIP kernel pass (gtls_transit_chi2_grouped: each point
+is read once and every model of its curve evaluated on it; wrapped as the
 torch.autograd.Function `TransitChi2`); for losses of the caller's own, the model flux of
 every point and its backward run in gtls_transit_fwd / gtls_transit_bwd (`TransitModel`).
 All three kernels are in csrc/transit_model.h; the parameter transform is ordinary torch
@@ -73,6 +74,17 @@ class ModelBatch:
     n_models: int
     y_pt: object = None  # float64 [total points]: flux per point (like t)
     w_pt: object = None  # float64 [total points]: weight per point
+    # the models grouped by curve for gtls_transit_chi2_grouped: grouped slot k holds model
+    # model_order[k] (None when the models already come curve by curve), model_rank its
+    # inverse; per curve its model and point-tile offsets; part, the tile-sum scratch
+    model_order: object = None
+    model_rank: object = None
+    model_off: object = None   # int64 [n_curves + 1]
+    tile_off: object = None    # int64 [n_curves + 1]
+    n_curves: int = 0
+    n_tiles: int = 0
+    max_models: int = 0
+    part: object = None        # float64 [n_tiles * max_models * 7]
 
     @classmethod
     def build(cls, curves, model_curve, device=0):
@@ -92,11 +104,29 @@ class ModelBatch:
         f64 = dict(dtype=torch.float64, device=dev)
         y_pt = np.concatenate(yc) if yc else np.zeros(0)
         w_pt = np.concatenate(wc) if wc else np.zeros(0)
+        # grouping by curve (stable), point tiles and the tile-sum scratch
+        n_curves = len(curves)
+        if len(model_curve) and (model_curve.min() < 0 or model_curve.max() >= n_curves):
+            raise ValueError("model_curve holds a curve index outside [0, %d)" % n_curves)
+        counts = np.bincount(model_curve, minlength=n_curves) if n_curves else np.zeros(0, int)
+        order = np.argsort(model_curve, kind="stable")
+        grouped = np.array_equal(order, np.arange(len(order)))
+        model_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        tp = int(_ffi.load().gtls_transit_tile_points())
+        tile_off = np.concatenate([[0], np.cumsum((n + tp - 1) // tp)]).astype(np.int64)
+        n_tiles = int(tile_off[-1])
+        max_models = int(counts.max(initial=0))
+        i64 = dict(dtype=torch.int64, device=dev)
         return cls(dev, torch.as_tensor(t, **f64), torch.as_tensor(pt_off, device=dev),
                    torch.as_tensor(model_curve, device=dev), torch.as_tensor(out_off, device=dev),
                    torch.as_tensor(y, **f64),
                    torch.as_tensor(w, **f64), int(n.max(initial=0)), len(model_curve),
-                   torch.as_tensor(y_pt, **f64), torch.as_tensor(w_pt, **f64))
+                   torch.as_tensor(y_pt, **f64), torch.as_tensor(w_pt, **f64),
+                   None if grouped else torch.as_tensor(order, **i64),
+                   None if grouped else torch.as_tensor(np.argsort(order), **i64),
+                   torch.as_tensor(model_off, **i64), torch.as_tensor(tile_off, **i64),
+                   n_curves, n_tiles, max_models,
+                   torch.empty(max(1, n_tiles * max_models * 7), **f64))
 
     @property
     def slots(self):
@@ -179,10 +209,12 @@ def transit_bwd(mb: ModelBatch, theta, grad_flux, mode="fp32c"):
     return out
 
 
-def transit_chi2(mb: ModelBatch, theta, mode="fp32c", grad=True):
+def transit_chi2(mb: ModelBatch, theta, mode="fp32c", grad=True, grouped=True):
     """Per-model chi2 = sum_i w_i (y_i - f_i)^2 over the model's curve (float64 [n_models])
     and, when grad, its gradient with respect to theta (float64 [n_models, 6]; else None),
-    in one fused pass (gtls_transit_chi2) that writes no flux slots."""
+    in one fused pass that writes no flux slots: by default gtls_transit_chi2_grouped (every
+    point read once, all models of its curve evaluated on it), with grouped=False
+    gtls_transit_chi2 (one workgroup per model)."""
     torch = _torch()
     theta = theta.detach().to(torch.float64).contiguous()
     if theta.shape != (mb.n_models, 6) or theta.device != mb.dev:
@@ -192,11 +224,24 @@ def transit_chi2(mb: ModelBatch, theta, mode="fp32c", grad=True):
     chi2 = torch.empty(mb.n_models, dtype=torch.float64, device=mb.dev)
     g = torch.empty((mb.n_models, 6), dtype=torch.float64, device=mb.dev) if grad else None
     ctx = _ctx(mb.dev)
-    code = _timed("chi2", lambda: _ffi.load().gtls_transit_chi2(
-        ctx.handle, MODES[mode], mb.n_models, _ffi.ptr(theta), _ffi.ptr(mb.model_curve),
-        _ffi.ptr(mb.pt_off), _ffi.ptr(mb.t), _ffi.ptr(mb.y_pt), _ffi.ptr(mb.w_pt),
-        _ffi.ptr(chi2), _ffi.ptr(g) if grad else None))
+    lib = _ffi.load()
+    if not grouped or mb.model_off is None:
+        code = _timed("chi2", lambda: lib.gtls_transit_chi2(
+            ctx.handle, MODES[mode], mb.n_models, _ffi.ptr(theta), _ffi.ptr(mb.model_curve),
+            _ffi.ptr(mb.pt_off), _ffi.ptr(mb.t), _ffi.ptr(mb.y_pt), _ffi.ptr(mb.w_pt),
+            _ffi.ptr(chi2), _ffi.ptr(g) if grad else None))
+        _ffi.check(ctx.handle, code)
+        return chi2, g
+    th = theta if mb.model_order is None else theta[mb.model_order].contiguous()
+    code = _timed("chi2", lambda: lib.gtls_transit_chi2_grouped(
+        ctx.handle, MODES[mode], mb.n_curves, mb.n_models, mb.max_models, _ffi.ptr(th),
+        _ffi.ptr(mb.model_off), _ffi.ptr(mb.pt_off), _ffi.ptr(mb.tile_off), mb.n_tiles,
+        _ffi.ptr(mb.t), _ffi.ptr(mb.y_pt), _ffi.ptr(mb.w_pt), _ffi.ptr(mb.part),
+        mb.part.numel(), _ffi.ptr(chi2), _ffi.ptr(g) if grad else None))
     _ffi.check(ctx.handle, code)
+    if mb.model_order is not None:
+        chi2 = chi2[mb.model_rank]
+        g = g[mb.model_rank] if grad else None
     return chi2, g
 
 

@@ -27,7 +27,7 @@
  *   gtls_spectra_batched
  *       spectra() of src/gputls/stats.py:95-122 (SR, power_raw, detrended power, SDE)
  *       with the running median of src/gputls/helpers.py:91-106, per light curve.
- *   gtls_transit_fwd / gtls_transit_bwd / gtls_transit_chi2
+ *   gtls_transit_fwd / gtls_transit_bwd / gtls_transit_chi2 / gtls_transit_chi2_grouped
  *       no reference counterpart: the model of the gradient refinement of grid peaks.
  *   gtls_set_timing / gtls_kernel_ms / gtls_set_tuning
  *       no reference counterpart: per-kernel timing and tuning of the search chain.
@@ -228,6 +228,24 @@ int gtls_transit_bwd(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double
 int gtls_transit_chi2(gtls_ctx* ctx, int32_t mode, int64_t n_models, const double* theta,
                       const int32_t* model_curve, const int64_t* pt_off, const double* t,
                       const double* y, const double* w, double* chi2, double* grad_theta);
+
+/*
+ * gtls_transit_chi2 with the models grouped by curve (no reference counterpart either):
+ * the models of curve c are [model_off[c], model_off[c + 1]) (at most max_models of them),
+ * theta in that order. The points of curve c are cut into tiles of
+ * gtls_transit_tile_points() consecutive points, tiles [tile_off[c], tile_off[c + 1]) with
+ * tile_off[c + 1] - tile_off[c] = ceil(N_c / tile points) and n_tiles = tile_off[n_curves];
+ * each tile is read once and every model of its curve is evaluated on it, so a call reads
+ * each point once whatever the number of models. part is caller scratch of part_len >=
+ * n_tiles * max_models * (grad_theta ? 7 : 1) doubles for the per-tile sums, which are
+ * added in tile order (results do not depend on the launch). Outputs as gtls_transit_chi2.
+ */
+int64_t gtls_transit_tile_points(void);
+int gtls_transit_chi2_grouped(gtls_ctx* ctx, int32_t mode, int32_t n_curves, int64_t n_models,
+                              int32_t max_models, const double* theta, const int64_t* model_off,
+                              const int64_t* pt_off, const int64_t* tile_off, int64_t n_tiles,
+                              const double* t, const double* y, const double* w, double* part,
+                              int64_t part_len, double* chi2, double* grad_theta);
 
 #ifdef __cplusplus
 }

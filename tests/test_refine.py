@@ -186,6 +186,45 @@ def test_fused_chi2_matches_the_flux_path(gpu):
 
 
 @pytest.mark.gpu
+def test_grouped_chi2_matches_the_per_model_kernel(gpu):
+    """Curves of exactly one point tile, one tile and a point, several tiles, and one with no
+    model; models listed out of curve order: the grouped kernel (each point read once) and
+    the one-workgroup-per-model kernel agree for every model, in both modes."""
+    from gtls_amd import _ffi
+    tp = int(_ffi.load().gtls_transit_tile_points())
+    rng = np.random.default_rng(11)
+    sizes = [tp, tp + 1, 3 * tp + 517, 700, 1500]
+    curves, thetas, mc = [], [], []
+    for c, n in enumerate(sizes):
+        t = np.sort(rng.uniform(200.0, 230.0, n))
+        truth = theta_row(1.7 + 0.4 * c, 200.4 + 0.05 * c, 0.1 + 0.01 * c, 3e-3)
+        y = R.transit_model_torch(truth, torch.tensor(t)[None])[0].numpy()
+        curves.append((t, y + rng.normal(0, 5e-4, n), rng.uniform(3e-4, 8e-4, n)))
+        if c == 3:
+            continue   # a curve without models
+        for j in range(1 + (c % 3) * 2):
+            th = truth[0].numpy().copy()
+            th[0] *= 1 + 1e-5 * j
+            th[1] += 0.004 * j
+            th[4], th[5] = 0.2 + 0.1 * j, 0.35
+            thetas.append(th)
+            mc.append(c)
+    perm = rng.permutation(len(mc))
+    mc = np.asarray(mc)[perm]
+    theta = torch.tensor(np.asarray(thetas)[perm], device="cuda")
+    mb = R.ModelBatch.build(curves, mc, 0)
+    assert mb.model_order is not None and mb.max_models == 5
+    for mode, tol in (("fp64", 1e-12), ("fp32c", 1e-11)):
+        c1, g1 = R.transit_chi2(mb, theta, mode, grouped=False)
+        c2, g2 = R.transit_chi2(mb, theta, mode)
+        scale = g1.abs().amax(dim=1, keepdim=True)
+        assert float(((c2 - c1).abs() / c1).max()) < tol, mode
+        assert float(((g2 - g1).abs() / scale).max()) < tol, mode
+        c3, none = R.transit_chi2(mb, theta, mode, grad=False)
+        assert none is None and float(((c3 - c1).abs() / c1).max()) < tol, mode
+
+
+@pytest.mark.gpu
 def test_autograd_function_chains_through_torch(gpu):
     mb, th, curves, mc, _ = model_batch(seed=2)
     theta = torch.tensor(th, device="cuda", requires_grad=True)
