@@ -130,25 +130,28 @@ def refine_leg(args, plans, batch0, skip):
            "curves_per_hour": round(len(curves) / dt * 3600.0, 1),
            "chi2_ratio_median": float(np.median(ratio)),
            "chi2_improved_models": "%d/%d" % (int(np.sum(ratio < 1)), len(ratio))}
-    # the kernels against the HBM roofline, bytes per model point and call: the fused loss
-    # kernel reads time, flux and weight (24 B); the flux kernels 8 B of time and 4 B of
-    # flux (forward: written; backward: the upstream gradient, read)
-    # (the fused kernel's models of one curve read the same points: unique bytes count each
-    # curve's points once per call)
+    # the kernels against the HBM roofline. The fused loss (gtls_transit_chi2_grouped: point
+    # tiles, every model of the tile's curve evaluated on them, then the in-order sum of the
+    # tile sums) reads each point's time, flux and weight (24 B) once per call; the flux
+    # kernels read 8 B of time and move 4 B of flux (forward: written; backward: the
+    # upstream gradient, read) per model point.
     unique_points = sum(len(c[0]) for c in curves)
-    for k, name, per in (("chi2", "transit_chi2_kernel", 24.0),
-                         ("fwd", "transit_fwd_kernel", 12.0), ("bwd", "transit_bwd_kernel", 12.0)):
+    ms, calls = kt.get("chi2", (0.0, 0))
+    if calls:
+        gbs = 24.0 * unique_points * calls / (ms * 1e-3) / 1e9
+        out["transit_chi2_grouped"] = {
+            "kernels": "transit_chi2_tile_kernel + transit_chi2_sum_kernel",
+            "ms_per_call": round(ms / calls, 4), "calls": calls,
+            "points_per_call": unique_points, "bytes_per_call": 24 * unique_points,
+            "hbm_GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "model_points_per_s": round(slots * calls / (ms * 1e-3), 1)}
+    for k, name in (("fwd", "transit_fwd_kernel"), ("bwd", "transit_bwd_kernel")):
         ms, calls = kt.get(k, (0.0, 0))
         if calls:
-            gbs = per * slots * calls / (ms * 1e-3) / 1e9
+            gbs = 12.0 * slots * calls / (ms * 1e-3) / 1e9
             out[name] = {"ms_per_call": round(ms / calls, 4), "calls": calls,
                          "algorithmic_GBps": round(gbs, 1),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
-            if k == "chi2":
-                ugbs = per * unique_points * calls / (ms * 1e-3) / 1e9
-                out[name].update({"unique_points": unique_points,
-                                  "unique_GBps": round(ugbs, 1),
-                                  "unique_hbm_frac": round(ugbs / HBM_PEAK_GBS, 4)})
     log("refine: %d models, %d steps in %.2f s (%.2f ms/step), median chi2 ratio %.4f" % (
         mb.n_models, args.refine_steps, dt, out["ms_per_step"], out["chi2_ratio_median"]))
     return out
@@ -225,6 +228,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tuning", default=None, metavar="TOEP_Q,CHUNK_MB,WG_PER_CU",
+                    help="apply this setting instead of the autotuner's choice")
     ap.add_argument("--no-full-check", action="store_true",
                     help="skip the float64 search of a whole batch (parity of every curve)")
     ap.add_argument("--eager", action="store_true",
@@ -289,20 +294,31 @@ def main():
         macs.sum(), time.time() - t0))
 
     tuning = None
-    if not args.no_tune:
+    if args.tuning:
+        setting = tuple(int(x) for x in args.tuning.split(","))
+        if len(setting) != 3:
+            print("--tuning takes TOEP_Q,CHUNK_MB,WG_PER_CU", file=sys.stderr)
+            sys.exit(2)
+        tune.set_tuning(0, *setting)
+        tuning = {"choice": {"toep_q": setting[0], "chunk_mb": setting[1],
+                             "wg_per_cu": setting[2]}, "source": "--tuning"}
+    elif not args.no_tune:
         ts = time.perf_counter()
         choice, times = tune.autotune(batches[0][1], skip, n_sample=32, repeats=1)
         tuning = {"choice": {"toep_q": choice[0], "chunk_mb": choice[1],
                              "wg_per_cu": choice[2]},
                   "candidates_ms": {"%d/%d/%d" % k: round(v, 2) for k, v in times.items()},
                   "seconds": round(time.perf_counter() - ts, 1),
-                  "sample": "32 curves of batch 0 spread over its sizes, one timed call each"}
+                  "sample": "32 curves of batch 0 spread over its sizes, one timed call per "
+                            "setting; coordinate search (workgroups/CU, chunk budget, "
+                            "Toeplitz bound in turn)"}
         log("autotune: toep_q %d, chunk %d MiB, %d workgroups/CU (%.1f s)" % (
             choice + (tuning["seconds"],)))
 
     # one graph per batch, all over one workspace (replayed in stream order), each with
     # the library's per-chunk kernel events captured
     graphs = []
+    ws = None
     if not args.eager:
         need = max(db.workspace_bytes("fp32c", skip) for _, db in batches)
         ws = torch.empty(need, dtype=torch.uint8, device="cuda")
@@ -579,6 +595,16 @@ def main():
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
+
+    # teardown in order: graphs and device buffers, then the library contexts, so that the
+    # process leaves nothing queued or allocated on the device when it exits
+    from gtls_amd import _ffi
+    del graphs, batches, ws
+    S._WORKSPACES.clear()
+    torch.cuda.synchronize()
+    _ffi.Context.close_all()
+    torch.cuda.empty_cache()
+    log("done")
 
 
 if __name__ == "__main__":
