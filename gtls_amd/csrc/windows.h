@@ -262,13 +262,13 @@ __device__ __forceinline__ void lane_take(LaneMin& L, double v, int32_t s, const
     L.sent_live = a.sentinel < v;
 }
 
-// Float copies of one width's constants for the estimates.
+// Float copies of one width's constants for the estimates (wss = w0 sum S^2, w2 = 2 w0).
 struct WidthConstsF {
-    float wc, ssk, os2, inv_d, depth_min;
+    float wss, w2, os2, inv_d, depth_min;
 };
 __device__ __forceinline__ WidthConstsF width_consts_f(const WidthArgs& a) {
-    return WidthConstsF{(float)a.wc, (float)a.ssk, (float)a.os2, (float)a.inv_d,
-                        (float)a.depth_min};
+    return WidthConstsF{(float)(a.wc * a.ssk), (float)(2.0 * a.wc), (float)a.os2,
+                        (float)a.inv_d, (float)a.depth_min};
 }
 
 // One candidate at position s with an open (gate) or closed window-mean gate, r = 2 m
@@ -281,8 +281,8 @@ __device__ __forceinline__ void tile_candidate(LaneMin& L, const WidthArgs& a,
                                                float sa, float sb, int32_t s) {
     if (gate) {
         if constexpr (ConstW) {
-            const float t1 = f.wc * (rs * rs) * f.ssk;
-            const float t2 = 2.f * f.wc * rs * sa;
+            const float t1 = f.wss * (rs * rs);
+            const float t2 = f.w2 * rs * sa;
             const float lo = (t1 - t2) - 1e-6f * (fabsf(t1) + fabsf(t2));
             if (!(lo >= L.thr)) {
                 const double v =
@@ -322,7 +322,13 @@ __device__ __forceinline__ void tile_candidate(LaneMin& L, const WidthArgs& a,
 // ---------------------------------------------------------------------------------------
 typedef float mfma_v4 __attribute__((ext_vector_type(4)));
 constexpr int kRingRows = 80;     // 64 rows of a super-tile + up to 15 look-ahead rows
-constexpr int kRingPitch = 84;    // column-major: column c at c * kRingPitch (16-byte rows of 4)
+// Column-major: column c at c * kRingPitch, rows of 4 bytes; rows 0 .. 15 are stored again at
+// kRingRows .. kRingRows + 15, so the up to 16 consecutive rows a lane's diagonal reads from
+// its first row never wrap (one base address, immediate offsets). 100 = 36 mod 64 banks:
+// the 16 columns of a 16-byte row store fall on distinct banks.
+constexpr int kRingMirror = 16;
+constexpr int kRingPitch = 100;
+static_assert(kRingPitch >= kRingRows + kRingMirror, "ring column too short for the mirror");
 constexpr int kRingFloats = 16 * kRingPitch;
 constexpr int kKGroup = 16;       // samples per K group (one 16x16x32 MFMA step over pairs)
 
@@ -368,30 +374,6 @@ struct HSweep {
     int32_t co;          // lane's B operand byte offset
     lds_char* LB;        // lane's staged B operand of group 0 (BL)
 };
-
-// The diagonal-sum sweep of super-tile st (64 G rows): A operand X[row q + b]: samples
-// b >= q meet zero template entries and rows past the data read block 0, so the sample
-// plane needs only finite values (the gather zero-fills its tail); B operand
-// C[c q_pad + b] for columns c <= F, the ones chunks F + 1 and F + 2 in columns 14 and 15
-// when `ones` (F <= 13), and the zero block (zrel bytes from C) in the other columns.
-__device__ __forceinline__ HSweep<4> h_sweep(const char* __restrict__ X, const char* __restrict__ C,
-                                             int32_t zrel, int32_t q, int32_t F, int64_t rows,
-                                             int64_t st, bool ones = false) {
-    const int lane = lane_id();
-    const int kk = lane >> 4, col = lane & 15;
-    const int qr = lane >> 2, qk = lane & 3;   // load lanes: row qr, K slice qk (a_permute)
-    HSweep<4> w;
-    w.X = X;
-    w.C = C;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int64_t row = 64 * st + 16 * t + qr;
-        w.xo[t] = (uint32_t)(4 * ((row < rows ? row * q : 0) + 4 * qk));
-    }
-    const int32_t ch = col <= F ? col : (ones && col >= kOnesCol ? F + 1 + (col - kOnesCol) : -1);
-    w.co = (ch >= 0) ? 4 * (ch * chunk_pad(q) + 4 * kk) : zrel + 16 * kk;
-    return w;
-}
 
 // One K group's operands of a sweep: the sample pairs of the NT tiles and the template
 // pairs (expanded into the two MFMA operands by h_mma).
@@ -528,24 +510,35 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 }
 
 // ---------------------------------------------------------------------------------------
-// Diagonal-sum sweeps over K-group pairs. The sample rows of the diagonal-sum tiles start
-// at arbitrary 4 q-byte offsets; one load per K group (16 rows x 64 bytes) fetches most
-// 128-byte cache lines twice, by two loads in flight together. Here a load covers 8 rows x
-// 128 bytes (two K groups) instead: lane l reads row 8 h + (l / 4) % 8 of the tile (h = the
-// tile's load 0 or 1), K group 2 p + l / 32, slice l % 4 (16 bytes). v_permlane32_swap then
-// gathers the first K group of all 16 rows into the load-0 register and the second into
-// the load-1 register (lanes 32 h + 4 r8 + s: row 8 h + r8, slice s), and the same
-// ds_bpermute as a_permute (source lane 4 r + s) moves them to the operand layout.
+// Diagonal-sum sweeps over K-group pairs: per tile, the sample operands of two K groups are
+// loaded together and run through their MFMAs as one step. Two load layouts (Direct):
+//  * Direct (common weight, one sample plane): each lane loads its operand in the MFMA
+//    layout itself (row l % 16 of the tile, K slice l / 16: 16 bytes), so one load touches
+//    16 rows, the 4 lanes of a row 16 apart.
+//  * Quad (per-sample weights, two planes per super-tile): a load covers 8 rows x 128 bytes
+//    (two K groups): lane l reads row 8 h + (l / 4) % 8 of the tile (h = the tile's load 0
+//    or 1), K group 2 p + l / 32, slice l % 4 (16 bytes), so most 128-byte lines are
+//    fetched once. v_permlane32_swap then gathers the first K group of all 16 rows into the
+//    load-0 register and the second into the load-1 register (lanes 32 h + 4 r8 + s: row
+//    8 h + r8, slice s), and ds_bpermute (source lane 4 r + s) moves them to the operand
+//    layout, 4 KiB per K group and wavefront through the CU's LDS crossbar.
+// Measured on the batched workload: Direct 6.4 % faster than Quad with a common weight (the
+// LDS was active 63 % of the kernel's cycles), 7.7 % slower with per-sample weights.
 // ---------------------------------------------------------------------------------------
 struct PSweep {
     const char* X;       // sample pair plane (wave-uniform)
     const char* C;       // template pair plane (wave-uniform)
-    int32_t rb;          // lane's row in tile 0, load 0: 64 st + (l / 4) % 8
+    int32_t rb;          // lane's row in tile 0 (load 0): 64 st + l % 16, Quad 64 st + (l / 4) % 8
     int32_t co;          // lane's B operand byte offset (K group 0)
     lds_char* LB;        // lane's staged B operand of group 0 (BL)
 };
 
-// Super-tile st's pair sweep over X and C (B operands and row rules as h_sweep).
+// Super-tile st's pair sweep over X and C: A operand X[row q + b] (samples b >= q meet zero
+// template entries and rows past the data read block 0, so the sample plane needs only
+// finite values: the gather zero-fills its tail); B operand C[c q_pad + b] for columns
+// c <= F, the ones chunks F + 1 and F + 2 in columns 14 and 15 when `ones` (F <= 13), and
+// the zero block (zrel bytes from C) in the other columns.
+template <bool Direct>
 __device__ __forceinline__ PSweep p_sweep(const char* __restrict__ X, const char* __restrict__ C,
                                           int32_t zrel, int32_t q, int32_t F, int64_t st,
                                           bool ones, lds_char* LB) {
@@ -554,7 +547,7 @@ __device__ __forceinline__ PSweep p_sweep(const char* __restrict__ X, const char
     PSweep w;
     w.X = X;
     w.C = C;
-    w.rb = (int32_t)(64 * st) + ((lane >> 2) & 7);
+    w.rb = (int32_t)(64 * st) + (Direct ? (lane & 15) : ((lane >> 2) & 7));
     const int32_t ch = col <= F ? col : (ones && col >= kOnesCol ? F + 1 + (col - kOnesCol) : -1);
     w.co = (ch >= 0) ? 4 * (ch * chunk_pad(q) + 4 * kk) : zrel + 16 * kk;
     w.LB = LB;
@@ -570,19 +563,32 @@ struct PSet {
 // Loads pair p (< ng_pairs) of a sweep (rows q samples apart, rows past `rows` read
 // block 0); the B operand of a second group past ng - 1 (odd ng) repeats group ng - 1 and
 // is not used.
-template <bool BL>
+template <bool BL, bool Direct>
 __device__ __forceinline__ void p_load(const PSweep& w, int32_t p, int32_t ng, int32_t q,
                                        int32_t rows, PSet& o) {
     const int lane = lane_id();
-    const uint32_t off = 128u * (uint32_t)p + 64u * (uint32_t)(lane >> 5) + 16u * (uint32_t)(lane & 3);
+    if constexpr (Direct) {
+        // K slice l / 16 of groups 2 p (h = 0) and 2 p + 1 (h = 1)
+        const uint32_t off = 128u * (uint32_t)p + 16u * (uint32_t)(lane >> 4);
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int32_t row = w.rb + 16 * t + 8 * h;
+        for (int t = 0; t < 4; ++t) {
+            const int32_t row = w.rb + 16 * t;
             const uint32_t ro = row < rows ? 4u * (uint32_t)row * (uint32_t)q : 0u;
-            o.a[t][h] = ld_h8(w.X + (ro + off));
+#pragma unroll
+            for (int h = 0; h < 2; ++h) o.a[t][h] = ld_h8(w.X + (ro + off + 64u * (uint32_t)h));
         }
+    } else {
+        const uint32_t off =
+            128u * (uint32_t)p + 64u * (uint32_t)(lane >> 5) + 16u * (uint32_t)(lane & 3);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int32_t row = w.rb + 16 * t + 8 * h;
+                const uint32_t ro = row < rows ? 4u * (uint32_t)row * (uint32_t)q : 0u;
+                o.a[t][h] = ld_h8(w.X + (ro + off));
+            }
+    }
     const int32_t g0 = 2 * p, g1 = min(2 * p + 1, ng - 1);
     if constexpr (BL) {
         o.b[0] = *(const __attribute__((address_space(3))) h8*)(w.LB + 1024 * g0);
@@ -593,22 +599,25 @@ __device__ __forceinline__ void p_load(const PSweep& w, int32_t p, int32_t ng, i
     }
 }
 
-// The operand layout of both K groups (permlane32 swap, then ds_bpermute), then their
-// MFMAs (the second group only when `two`).
+// The MFMAs of both K groups (the second only when `two`); Quad loads first move the
+// operands to the MFMA layout (permlane32 swap, then ds_bpermute from lane paddr / 4).
+template <bool Direct>
 __device__ __forceinline__ void p_mma(mfma_v4 (&acc)[4], PSet& o, int paddr, bool two) {
+    if constexpr (!Direct) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        uint32_t v0[4], v1[4];
-        __builtin_memcpy(v0, &o.a[t][0], sizeof(v0));
-        __builtin_memcpy(v1, &o.a[t][1], sizeof(v1));
+        for (int t = 0; t < 4; ++t) {
+            uint32_t v0[4], v1[4];
+            __builtin_memcpy(v0, &o.a[t][0], sizeof(v0));
+            __builtin_memcpy(v1, &o.a[t][1], sizeof(v1));
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const auto sw = __builtin_amdgcn_permlane32_swap(v0[e], v1[e], false, false);
-            v0[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(paddr, (int)sw[0]);
-            v1[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(paddr, (int)sw[1]);
+            for (int e = 0; e < 4; ++e) {
+                const auto sw = __builtin_amdgcn_permlane32_swap(v0[e], v1[e], false, false);
+                v0[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(paddr, (int)sw[0]);
+                v1[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(paddr, (int)sw[1]);
+            }
+            __builtin_memcpy(&o.a[t][0], v0, sizeof(v0));
+            __builtin_memcpy(&o.a[t][1], v1, sizeof(v1));
         }
-        __builtin_memcpy(&o.a[t][0], v0, sizeof(v0));
-        __builtin_memcpy(&o.a[t][1], v1, sizeof(v1));
     }
     HSet<4> g;
     g.b = o.b[0];
@@ -626,7 +635,7 @@ __device__ __forceinline__ void p_mma(mfma_v4 (&acc)[4], PSet& o, int paddr, boo
 
 // One pair sweep into acc (zeroed here), pipelined as run_sweep over pairs: `cur` with its
 // pair 0 already in s0; on return s0 holds pair 0 of `nxt`.
-template <bool BL>
+template <bool BL, bool Direct>
 __device__ __forceinline__ void run_psweep(mfma_v4 (&acc)[4], const PSweep& cur,
                                            const PSweep& nxt, int32_t ng, int32_t q,
                                            int32_t rows, PSet& s0, PSet& s1) {
@@ -637,9 +646,9 @@ __device__ __forceinline__ void run_psweep(mfma_v4 (&acc)[4], const PSweep& cur,
     for (int t = 0; t < 4; ++t) acc[t] = mfma_v4{0.f, 0.f, 0.f, 0.f};
     int32_t p = 0;
     for (; p + 1 < np; p += 2) {
-        p_load<BL>(cur, p + 1, ng, q, rows, s1);
+        p_load<BL, Direct>(cur, p + 1, ng, q, rows, s1);
         __builtin_amdgcn_sched_barrier(0);
-        p_mma(acc, s0, paddr, true);
+        p_mma<Direct>(acc, s0, paddr, true);
         __builtin_amdgcn_sched_barrier(0);
         // pair p + 2 of this sweep, or pair 0 of the next one
         const bool more = p + 2 < np;
@@ -649,20 +658,20 @@ __device__ __forceinline__ void run_psweep(mfma_v4 (&acc)[4], const PSweep& cur,
         w.rb = more ? cur.rb : nxt.rb;
         w.co = more ? cur.co : nxt.co;
         w.LB = more ? cur.LB : nxt.LB;
-        p_load<BL>(w, more ? p + 2 : 0, ng, q, rows, s0);
+        p_load<BL, Direct>(w, more ? p + 2 : 0, ng, q, rows, s0);
         __builtin_amdgcn_sched_barrier(0);
-        p_mma(acc, s1, paddr, !(odd && p + 2 == np));
+        p_mma<Direct>(acc, s1, paddr, !(odd && p + 2 == np));
         __builtin_amdgcn_sched_barrier(0);
     }
     if (p < np) {   // an odd pair count: the last pair is in s0
-        p_mma(acc, s0, paddr, !odd);
+        p_mma<Direct>(acc, s0, paddr, !odd);
         __builtin_amdgcn_sched_barrier(0);
-        p_load<BL>(nxt, 0, ng, q, rows, s0);
+        p_load<BL, Direct>(nxt, 0, ng, q, rows, s0);
     }
 }
 
 // The pair sweep `cur` of a super-tile, written to the ring (see run_psweep).
-template <bool BL>
+template <bool BL, bool Direct>
 __device__ __forceinline__ void super_tile_prows(const PSweep& cur, const PSweep& nxt, int32_t ng,
                                                  int32_t q, int32_t rows, PSet& s0, int ring0,
                                                  float* ring) {
@@ -670,32 +679,15 @@ __device__ __forceinline__ void super_tile_prows(const PSweep& cur, const PSweep
     const int kk = lane >> 4, col = lane & 15;
     mfma_v4 acc[4];
     PSet s1;
-    run_psweep<BL>(acc, cur, nxt, ng, q, rows, s0, s1);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        int r = ring0 + 16 * t + 4 * kk;
-        r = r >= kRingRows ? r - kRingRows : r;
-        *(mfma_v4*)(ring + col * kRingPitch + r) = acc[t];
-    }
-}
-
-// The sweep `cur` of a super-tile, written to the ring (see run_sweep).
-template <bool BL>
-__device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSweep<4>& nxt,
-                                                int32_t ng, HSet<4>& s0,
-                                                int ring0, float* ring) {
-    const int lane = lane_id();
-    const int kk = lane >> 4, col = lane & 15;
-    mfma_v4 acc[4], unused[4];
-    HSet<4> s1;
-    run_sweep<4, false, BL>(acc, unused, cur, nxt, ng, s0, s1);
+    run_psweep<BL, Direct>(acc, cur, nxt, ng, q, rows, s0, s1);
     // rows 16 t + 4 kk .. + 3 of column col: one 16-byte store (ring0 is a multiple of 16
-    // and kRingRows of 4, so the four rows never wrap apart)
+    // and kRingRows of 4, so the four rows never wrap apart), again at the mirror rows
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         int r = ring0 + 16 * t + 4 * kk;
         r = r >= kRingRows ? r - kRingRows : r;
         *(mfma_v4*)(ring + col * kRingPitch + r) = acc[t];
+        if (r < kRingMirror) *(mfma_v4*)(ring + col * kRingPitch + kRingRows + r) = acc[t];
     }
 }
 
@@ -706,8 +698,8 @@ __device__ __forceinline__ void super_tile_rows(const HSweep<4>& cur, const HSwe
 // 16 l (S plane; the S^2 plane follows after ng groups).
 constexpr int kWaveLdsBytes = 20480;
 constexpr int kRingBytes = 4 * kRingFloats;
-constexpr int kStageGroupsConstW = (kWaveLdsBytes - kRingBytes) / 1024;        // 14
-constexpr int kStageGroupsWeighted = (kWaveLdsBytes - 2 * kRingBytes) / 2048;  // 4
+constexpr int kStageGroupsConstW = (kWaveLdsBytes - kRingBytes) / 1024;        // 13
+constexpr int kStageGroupsWeighted = (kWaveLdsBytes - 2 * kRingBytes) / 2048;  // 3
 __shared__ __attribute__((aligned(16))) float g_wave_lds[4 * kWaveLdsBytes / 4];
 
 // Sample pair planes of one (period, width) item of the matrix-core paths: the A-sum
@@ -750,7 +742,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     // the staged operands: after the ring(s); lane's slot of group 0 of each plane
     lds_char* stage = (lds_char*)(ringA + (ConstW ? 1 : 2) * kRingFloats) + 16 * lane;
     auto sweep = [&](const char* Xp, const char* Cp, int64_t st, bool ones, int plane) {
-        return p_sweep(Xp, Cp, zrel, q, F, st, ones, stage + 1024 * ng * plane);
+        return p_sweep<ConstW>(Xp, Cp, zrel, q, F, st, ones, stage + 1024 * ng * plane);
     };
     if constexpr (BL) {
         // every lane's B operand of each K group (and plane), 4 groups of loads in flight
@@ -773,10 +765,10 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     // sweep is loaded during the one before (past the last super-tile: row block 0 again)
     PSet ops;
     const int32_t rows32 = (int32_t)rows;
-    p_load<BL>(sweep(P.X, C, 0, TileMean, 0), 0, ng, q, rows32, ops);
+    p_load<BL, ConstW>(sweep(P.X, C, 0, TileMean, 0), 0, ng, q, rows32, ops);
     for (int64_t st = 0; st < nsuper; ++st) {
-        const int64_t kc = 64 * st - F + lane;
-        const bool cand = kc >= 0 && kc < K;
+        const int32_t kc = (int32_t)(64 * st) - F + lane;   // K <= N <= 2^30
+        const bool cand = kc >= 0 && kc < (int32_t)K;
         // the window-mean prefix sums of this super-tile's candidates, requested early
         double dlo = 0.0, dhi = 0.0;
         if constexpr (!TileMean) {
@@ -789,47 +781,59 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
         const PSweep wa = sweep(P.X, C, st, TileMean, 0);
         if constexpr (ConstW) {
-            super_tile_prows<BL>(wa, sweep(P.X, C, stn, TileMean, 0), ng, q, rows32, ops, ring0,
+            super_tile_prows<BL, true>(wa, sweep(P.X, C, stn, TileMean, 0), ng, q, rows32, ops,
+                                       ring0,
                                  ringA);
         } else {
             const PSweep wb = sweep(P.W, C2, st, false, 1);
-            super_tile_prows<BL>(wa, wb, ng, q, rows32, ops, ring0, ringA);
-            super_tile_prows<BL>(wb, sweep(P.X, C, stn, false, 0), ng, q, rows32, ops, ring0,
+            super_tile_prows<BL, false>(wa, wb, ng, q, rows32, ops, ring0, ringA);
+            super_tile_prows<BL, false>(wb, sweep(P.X, C, stn, false, 0), ng, q, rows32, ops,
+                                        ring0,
                                  ringB);
         }
         wave_sync_lds();
-        // diagonal sums: all reads issue before the first add, columns past F masked
-        float va[16], vb[16], vu[15];
-        const int r0 = ring0 - F + lane;
+        // diagonal sums: column c at the lane's row rw + min(c, F) (rows rw .. rw + F, this
+        // super-tile's and the previous one's, mirrored: no wrap), all reads issued before
+        // the first add. Columns F + 1 .. 15 of the A and B sums met the zero block, so
+        // their entries in row rw + F are exact zeros and are added unmasked (the ones
+        // columns 14 and 15 of TileMean are left out): the same terms in the same order.
+        // Rows past rw + F can still hold the previous item's values; the block sums read
+        // there (c >= F) are masked.
+        constexpr int NA = TileMean ? kOnesCol : 16;
+        constexpr int NU = kOnesCol - 2;   // TileMean: F <= 12 block sums
+        const int rw = ring_wrap(ring0 - F + lane);
+        const float* ra = ringA + rw;
+        const float* rb = ringB + rw;
+        float va[NA], vb[ConstW ? 1 : 16], vu[TileMean ? NU + 1 : 1];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const int r = ring_wrap(r0 + min(c, F));
-            va[c] = ringA[c * kRingPitch + r];
-            if constexpr (!ConstW) vb[c] = ringB[c * kRingPitch + r];
+        for (int c = 0; c < NA; ++c) va[c] = ra[c * kRingPitch + min(c, F)];
+        if constexpr (!ConstW) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) vb[c] = rb[c * kRingPitch + min(c, F)];
         }
         if constexpr (TileMean) {
             // block sums (column 14) of blocks k .. k + F - 1, the rem sum (15) of block k + F
 #pragma unroll
-            for (int c = 0; c < 14; ++c)
-                vu[c] = ringA[kOnesCol * kRingPitch + ring_wrap(r0 + min(c, F))];
-            vu[14] = ringA[(kOnesCol + 1) * kRingPitch + ring_wrap(r0 + F)];
+            for (int c = 0; c < NU; ++c) vu[c] = ra[kOnesCol * kRingPitch + c];
+            vu[NU] = ra[(kOnesCol + 1) * kRingPitch + F];
         }
         float sa = 0.f, sb = 0.f, su = 0.f;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            sa += c <= F ? va[c] : 0.f;
-            if constexpr (!ConstW) sb += c <= F ? vb[c] : 0.f;
+        for (int c = 0; c < NA; ++c) sa += va[c];
+        if constexpr (!ConstW) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) sb += vb[c];
         }
         if constexpr (TileMean) {
 #pragma unroll
-            for (int c = 0; c < 14; ++c) su += c < F ? vu[c] : 0.f;
-            su += vu[14];
+            for (int c = 0; c < NU; ++c) su += c < F ? vu[c] : 0.f;
+            su += vu[NU];
         }
         sa *= P.inv_x;
         if constexpr (!ConstW) sb *= P.inv_w;
         wave_sync_lds();
         if (cand) {
-            const int32_t s = (int32_t)(kc * q);
+            const int32_t s = kc * q;
             if constexpr (TileMean) {
                 const float m = su * inv_xd;
                 tile_candidate<ConstW>(L, a, fc, m > fc.depth_min, m * fc.os2, sa, sb, s);
@@ -921,15 +925,18 @@ __device__ __forceinline__ ArgMin toep_width(const HPlanes& P, const char* __res
             run_sweep<NT, false>(acc2, unused, sweep(P.W, T2, st), sweep(P.X, T, stn), ng, s0,
                                  s1);
         }
-        // the statistic of the lane's 4 NT candidates
+        // the statistic of the lane's 4 NT candidates (indices and positions fit int32:
+        // K <= N <= 2^30)
+        const int32_t mbase = (int32_t)(kToepStep * st) + 64 * kk + col;
+        const int32_t K32 = (int32_t)K;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int e = 4 * t + v;
-                const int64_t m = kToepStep * st + 256 * t + 16 * (4 * kk + v) + col;
-                if (m < K) {
-                    const int32_t s = (int32_t)(m * q);
+                const int32_t m = mbase + 256 * t + 16 * v;
+                if (m < K32) {
+                    const int32_t s = m * q;
                     const float sa = acc[t][v] * P.inv_x;
                     if constexpr (ConstW) {
                         const float mm = acc2[t][v] * inv_xd;
