@@ -720,7 +720,10 @@ struct HPlanes {
 // prefix sums of u. With BL the template operands of every K group are staged once in the
 // wavefront's LDS (ng <= kStageGroupsConstW, or kStageGroupsWeighted for both planes)
 // instead of being read from memory by every super-tile.
-template <bool ConstW, bool TileMean, bool BL>
+// FC: the chunk count F = d / q when known at compile time (8: every width with q >= 8 at
+// the default skip of 8), so that the diagonal reads take immediate offsets and the columns
+// past F are not read at all; 0 for any F.
+template <bool ConstW, bool TileMean, bool BL, int FC = 0>
 __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __restrict__ C,
                                              const char* __restrict__ C2, int32_t zrel,
                                              const double* __restrict__ D,
@@ -732,7 +735,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     float* ringB = ringA + kRingFloats;
     const int32_t d = a.d, q = a.q;
     const int64_t K = a.K;
-    const int32_t F = d / q;
+    const int32_t F = FC ? FC : d / q;
     const int64_t rows = K + F;
     const int64_t nsuper = (rows + 63) / 64;
     LaneMin L = lane_min_init();
@@ -799,7 +802,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
         // columns 14 and 15 of TileMean are left out): the same terms in the same order.
         // Rows past rw + F can still hold the previous item's values; the block sums read
         // there (c >= F) are masked.
-        constexpr int NA = TileMean ? kOnesCol : 16;
+        constexpr int NA = FC ? FC + 1 : (TileMean ? kOnesCol : 16);   // columns read
         constexpr int NU = kOnesCol - 2;   // TileMean: F <= 12 block sums
         const int rw = ring_wrap(ring0 - F + lane);
         const float* ra = ringA + rw;
