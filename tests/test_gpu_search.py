@@ -117,13 +117,28 @@ def test_search_parity_with_dy_and_gaps(gpu, mode):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "fp32c"])
-def test_search_parity_single_2min_subset(gpu, mode):
-    """The 19,728-point 2-min curve: every 7th period of its grid (the oracle's full grid
-    runs in the CPU-baseline leg of bench.py)."""
+def test_search_parity_single_2min_full_grid(gpu, mode):
+    """The 19,728-point 2-min curve on its whole period grid at the default T0_fit_margin
+    0.125 (the float32 mode of it is the precision configuration), and on every 25th period
+    with no skip (T0_fit_margin 0)."""
+    from parity_util import check, report
     t, y, dy, truth = synthetic.single_2min()
     plan = plan_for(t, y, dy)
-    sel = np.arange(0, len(plan.periods), 7)
-    assert_parity(plan, mode, 8, plan.periods[sel], plan.k_lo[sel], plan.k_hi[sel])
+    skip = skip_factor(0.125)
+    c, k, s = [x.cpu().numpy() for x in S.DeviceCurve(plan).search(
+        plan.periods, plan.k_lo, plan.k_hi, mode=mode, skip_point=skip)]
+    worst, n_same, n_ok = check(plan, mode, skip, plan.periods, plan.k_lo, plan.k_hi, c, k, s,
+                                "C1")
+    report("C1-2min" if mode == "fp64" else "C3-2min-precision", mode, worst, n_same, n_ok,
+           "n=%d skip=%d" % (plan.n, skip))
+    sel = np.arange(0, len(plan.periods), 25)
+    per, klo, khi = plan.periods[sel], plan.k_lo[sel], plan.k_hi[sel]
+    none = skip_factor(0.0)
+    c, k, s = [x.cpu().numpy() for x in S.DeviceCurve(plan).search(per, klo, khi, mode=mode,
+                                                                   skip_point=none)]
+    worst, n_same, n_ok = check(plan, mode, none, per, klo, khi, c, k, s, "C1 no skip")
+    report("C1-2min-no-skip" if mode == "fp64" else "C3-2min-no-skip", mode, worst, n_same,
+           n_ok, "n=%d skip=none" % plan.n)
 
 
 def test_edge_cases(gpu):
