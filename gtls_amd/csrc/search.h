@@ -291,7 +291,7 @@ __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw
 // thread's gathers are in flight before its first store. All threads call it; no barrier.
 __device__ __forceinline__ void gather_window_pairs(const float2* __restrict__ uw, int64_t o0,
                                                     int cnt, const int32_t* sidx, uint32_t* hb,
-                                                    float sx) {
+                                                    float sx, int64_t n, int64_t wrap) {
     float u[kWinPer];
 #pragma unroll
     for (int e = 0; e < kWinPer; ++e) {
@@ -301,7 +301,12 @@ __device__ __forceinline__ void gather_window_pairs(const float2* __restrict__ u
 #pragma unroll
     for (int e = 0; e < kWinPer; ++e) {
         const int j = threadIdx.x + e * kBlock;
-        if (j < cnt) __builtin_nontemporal_store(split_pair(u[e] * sx), &hb[o0 + j]);
+        if (j < cnt) {
+            const uint32_t v = split_pair(u[e] * sx);
+            __builtin_nontemporal_store(v, &hb[o0 + j]);
+            // the wrap-around extension: position n + p repeats position p < wrap
+            if (o0 + j < wrap) __builtin_nontemporal_store(v, &hb[n + o0 + j]);
+        }
     }
 }
 
@@ -384,13 +389,17 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
             const R2* uw = p.uw0 + pt0;
             double carry = 0.0;
             auto put = [&](int64_t j, Real u, Real w) { sink.put(j, u, w); };
+            // float32 with a common weight and no running sums: the windowed sink writes the
+            // wrap-around copy itself when it needs one lap (dmax <= n)
+            const int64_t wrap_in_sink =
+                (sizeof(Real) == 4 && constw && !D && dmax <= n) ? (int64_t)dmax : 0;
             const bool windowed = wg_fold_sort_windows(
                 p.t + pt0, n, P, p.negative_time != 0, perm, hist, red_i, s_counts, s_phi,
                 s_idx, s_mids, s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                     if constexpr (sizeof(Real) == 4) {
                         if (constw && !D) {
                             gather_window_pairs((const float2*)uw, wstart, wcnt, s_idx, sink.hb,
-                                                sink.sx);
+                                                sink.sx, n, wrap_in_sink);
                             return;
                         }
                         if (constw) {
@@ -411,7 +420,8 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
             }
             __syncthreads();
             // the wrap-around extension: positions n .. np - 1 repeat positions 0 .. dmax - 1
-            for (int64_t j = n + tid; j < np_; j += kBlock) {
+            // (already written by the sink when it took it; not when the global form ran)
+            for (int64_t j = n + tid + ((windowed && wrap_in_sink) ? np_ - n : 0); j < np_; j += kBlock) {
                 const int64_t jj = (j - n) % n;
                 const int64_t laps = (j - n) / n + 1;
                 sink.copy(j, jj);
