@@ -3,7 +3,10 @@
 //
 // Fold: phase = x - trunc(x), x = t / P in IEEE float64 (the fold of
 // src/gputls/GPUFun.cu:28-39); the quotient is correctly rounded, the subtraction exact,
-// so the keys equal the reference's bit for bit.
+// so the keys equal the reference's bit for bit. The windowed form forms the quotient from
+// the period's reciprocal (fold_math.h: the same phases); a curve with a time below 2^-960
+// in magnitude, or a period outside [2^-1000, 2^1000], takes the global form, which
+// divides.
 //
 // Sort: the order of (phase, index) pairs -- the permutation a stable argsort of the
 // phases gives (src/gputls/core.py:775-777). One most-significant-digit radix pass
@@ -24,6 +27,7 @@
 // their bucket positions, orders every bucket there and hands the window to a sink. A
 // period whose largest bucket exceeds kWaveSortMax keys takes the global form above.
 #pragma once
+#include "fold_math.h"
 #include "wg_primitives.h"
 
 namespace gtls {
@@ -38,6 +42,13 @@ __host__ __device__ inline int buckets_for(int64_t n) {
 
 __device__ __forceinline__ double fold_phase(double t, double P) {
     const double x = t / P;
+    return x - trunc(x);
+}
+
+// The same phase with the quotient from rP = fold_reciprocal(P), for times that are zero
+// or at least 2^-960 in magnitude.
+__device__ __forceinline__ double fold_phase_r(double t, double P, double rP) {
+    const double x = fold_quotient(t, P, rP);
     return x - trunc(x);
 }
 
@@ -419,9 +430,10 @@ constexpr int kMaxWin = 255;                         // windows of one period (u
 // Runs with 256 threads (kWinPer staged samples per thread).
 // sink(wstart, wcnt) is called by every thread for each window: the samples of sorted
 // positions wstart .. wstart + wcnt - 1 sit in sphi / sidx[0 .. wcnt); it may use
-// barriers. Returns false (every thread) when a bucket holds more than kWaveSortMax keys
-// or the period needs more than kMaxWin windows: nothing was sorted, the caller takes the
-// global form.
+// barriers. Returns false (every thread) when a bucket holds more than kWaveSortMax keys,
+// the period needs more than kMaxWin windows, or the reciprocal form of the fold does not
+// apply (a time of the curve below 2^-960 in magnitude, a period outside [2^-1000,
+// 2^1000]): nothing was sorted, the caller takes the global form.
 template <class Sink>
 __device__ __forceinline__ bool wg_fold_sort_windows(
     const double* __restrict__ t, int64_t n, double P, bool neg,
@@ -429,10 +441,12 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     int32_t* sidx, int64_t* mids, uint8_t* wtab, int* wb, int* ws, int* wcur, Sink sink) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int B = buckets_for(n);
+    const double rP = fold_reciprocal(P);
     for (int b = tid; b < B; b += nt) hist[hslot(b)] = 0;
     if (tid < 4) counts[tid] = 0;
     __syncthreads();
     constexpr int U = 8;   // samples in flight per thread
+    bool small = rP != rP;
     for (int64_t base = 0; base < n; base += (int64_t)U * nt) {
         double tv[U];
 #pragma unroll
@@ -444,10 +458,12 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         for (int u = 0; u < U; ++u) {
             const int64_t j = base + (int64_t)u * nt + tid;
             if (j < n) {
-                atomicAdd(&hist[hslot(phase_bucket(fold_phase(tv[u], P), B, neg))], 1);
+                small |= fold_time_small(tv[u]);
+                atomicAdd(&hist[hslot(phase_bucket(fold_phase_r(tv[u], P, rP), B, neg))], 1);
             }
         }
     }
+    if (small) atomicOr(&counts[0], 2);
     __syncthreads();
     // exclusive scan of the histogram; flag a bucket too large for the windowed form
     {
@@ -469,7 +485,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         if (mx > kWaveSortMax) atomicOr(&counts[0], 1);
     }
     __syncthreads();
-    // Every thread reads the flag before any thread leaves: the caller's global form resets
+    // Every thread reads the flags before any thread leaves: the caller's global form resets
     // counts[] before its first barrier, so a late read after a return could see 0.
     {
         const bool too_big = counts[0] != 0;
@@ -561,7 +577,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
 #pragma unroll
             for (int u = 0; u < UW; ++u) {
                 const int64_t j = base + (int64_t)u * nt + tid;
-                bv[u] = j < n ? phase_bucket(fold_phase(tq[u], P), B, neg) : -1;
+                bv[u] = j < n ? phase_bucket(fold_phase_r(tq[u], P, rP), B, neg) : -1;
             }
 #pragma unroll
             for (int u = 0; u < UW; ++u) {
@@ -591,7 +607,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
 #pragma unroll
             for (int u = 0; u < kWinPer; ++u) {
                 if (jj[u] >= 0) {
-                    const double ph = fold_phase(tv[u], P);
+                    const double ph = fold_phase_r(tv[u], P, rP);
                     const int pos = atomicAdd(&hist[hslot(phase_bucket(ph, B, neg))], 1) - wstart;
                     sphi[pos] = ph;
                     sidx[pos] = (int32_t)jj[u];

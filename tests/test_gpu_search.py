@@ -83,6 +83,11 @@ def test_fold_sort_bit_exact(gpu):
         (np.repeat(1000 + np.arange(8) / 48.0, 600), np.array([0.5, 0.77])),
         # one time repeated: a single bucket of 5000 equal phases, ordered by index
         (np.full(5000, 1234.5), np.array([0.7, 3.1])),
+        # times below 2^-960 (the reciprocal form of the fold would not apply: the dividing
+        # global form), zeros of both signs, and periods past 2^1000
+        (np.concatenate([[-0.0, 0.0, 3e-310, 1e-300], 1000 + np.arange(4000) / 720.0]),
+         np.array([0.5, 1.37, 2.0 ** 1001])),
+        (1000 + np.arange(4000) / 720.0, np.array([2.0 ** 1001, 2.0 ** -1001])),
     ]
     for t, periods in cases:
         got = S.fold_sort(t, periods).cpu().numpy()
@@ -139,6 +144,18 @@ def test_search_parity_single_2min_full_grid(gpu, mode):
     worst, n_same, n_ok = check(plan, mode, none, per, klo, khi, c, k, s, "C1 no skip")
     report("C1-2min-no-skip" if mode == "fp64" else "C3-2min-no-skip", mode, worst, n_same,
            n_ok, "n=%d skip=none" % plan.n)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32c"])
+def test_tiny_time_takes_the_dividing_form(gpu, mode):
+    """A curve with a time below 2^-960: the windowed sort's reciprocal fold does not
+    apply, so every period of it is sorted by the global form, which divides."""
+    t, y = small_curve(1200, 5)
+    periods = plan_for(t, y).periods
+    plan = plan_for(np.concatenate([[1e-300], t]), np.concatenate([[1.0], y]),
+                    periods=periods)
+    sel = np.arange(0, len(plan.periods), 7)
+    assert_parity(plan, mode, 8, plan.periods[sel], plan.k_lo[sel], plan.k_hi[sel])
 
 
 def test_edge_cases(gpu):
