@@ -545,7 +545,12 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
             const int32_t zrel = (int32_t)(4 * (p.zoff - co));
             // template operands staged in LDS when they fit
             const int32_t ng = chunk_pad(q) / kKGroup;
-            if (constw && F == 8) {
+            if (constw && F == 8 && q >= kQuadQ) {
+                if (ng <= kStageGroupsConstW)
+                    w = mfma_width<true, true, true, 8, false>(hp, C, nullptr, zrel, D, a);
+                else
+                    w = mfma_width<true, true, false, 8, false>(hp, C, nullptr, zrel, D, a);
+            } else if (constw && F == 8) {
                 if (ng <= kStageGroupsConstW)
                     w = mfma_width<true, true, true, 8>(hp, C, nullptr, zrel, D, a);
                 else

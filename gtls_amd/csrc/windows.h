@@ -512,18 +512,24 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 // ---------------------------------------------------------------------------------------
 // Diagonal-sum sweeps over K-group pairs: per tile, the sample operands of two K groups are
 // loaded together and run through their MFMAs as one step. Two load layouts (Direct):
-//  * Direct (common weight, one sample plane): each lane loads its operand in the MFMA
+//  * Direct (common weight, strides below kQuadQ): each lane loads its operand in the MFMA
 //    layout itself (row l % 16 of the tile, K slice l / 16: 16 bytes), so one load touches
-//    16 rows, the 4 lanes of a row 16 apart.
-//  * Quad (per-sample weights, two planes per super-tile): a load covers 8 rows x 128 bytes
-//    (two K groups): lane l reads row 8 h + (l / 4) % 8 of the tile (h = the tile's load 0
-//    or 1), K group 2 p + l / 32, slice l % 4 (16 bytes), so most 128-byte lines are
-//    fetched once. v_permlane32_swap then gathers the first K group of all 16 rows into the
-//    load-0 register and the second into the load-1 register (lanes 32 h + 4 r8 + s: row
-//    8 h + r8, slice s), and ds_bpermute (source lane 4 r + s) moves them to the operand
-//    layout, 4 KiB per K group and wavefront through the CU's LDS crossbar.
-// Measured on the batched workload: Direct 6.4 % faster than Quad with a common weight (the
-// LDS was active 63 % of the kernel's cycles), 7.7 % slower with per-sample weights.
+//    16 rows of 64 bytes, the 4 lanes of a row 16 apart.
+//  * Quad (per-sample weights; common weight from stride kQuadQ on): a load covers 8 rows x
+//    128 bytes (two K groups): lane l reads row 8 h + (l / 4) % 8 of the tile (h = the
+//    tile's load 0 or 1), K group 2 p + l / 32, slice l % 4 (16 bytes), so most 128-byte
+//    lines are fetched once. v_permlane32_swap then gathers the first K group of all 16
+//    rows into the load-0 register and the second into the load-1 register (lanes
+//    32 h + 4 r8 + s: row 8 h + r8, slice s), and ds_bpermute (source lane 4 r + s) moves
+//    them to the operand layout, 4 KiB per K group and wavefront through the CU's LDS
+//    crossbar.
+// Rows are 4 q bytes apart. Measured on the batched workload with a common weight: Direct
+// everywhere 6.4 % faster than Quad everywhere (with Quad the LDS was active 63 % of the
+// kernel's cycles); Quad from q = 96 on (where the sweeps are long and the operand moves
+// few per candidate) another 6.6 % off the kernel's time on batch 0 (32: 2.1 %, 64: 4.7 %,
+// 128: 6.4 %, 192: 5.5 %, 256: 4.5 %). With per-sample weights Quad everywhere (Direct
+// 7.7 % slower).
+constexpr int kQuadQ = 96;
 // ---------------------------------------------------------------------------------------
 struct PSweep {
     const char* X;       // sample pair plane (wave-uniform)
@@ -722,8 +728,9 @@ struct HPlanes {
 // instead of being read from memory by every super-tile.
 // FC: the chunk count F = d / q when known at compile time (8: every width with q >= 8 at
 // the default skip of 8), so that the diagonal reads take immediate offsets and the columns
-// past F are not read at all; 0 for any F.
-template <bool ConstW, bool TileMean, bool BL, int FC = 0>
+// past F are not read at all; 0 for any F. Direct: the sample operands' load layout (see
+// p_load; per-sample weights always take Quad).
+template <bool ConstW, bool TileMean, bool BL, int FC = 0, bool Direct = ConstW>
 __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __restrict__ C,
                                              const char* __restrict__ C2, int32_t zrel,
                                              const double* __restrict__ D,
@@ -745,7 +752,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     // the staged operands: after the ring(s); lane's slot of group 0 of each plane
     lds_char* stage = (lds_char*)(ringA + (ConstW ? 1 : 2) * kRingFloats) + 16 * lane;
     auto sweep = [&](const char* Xp, const char* Cp, int64_t st, bool ones, int plane) {
-        return p_sweep<ConstW>(Xp, Cp, zrel, q, F, st, ones, stage + 1024 * ng * plane);
+        return p_sweep<Direct>(Xp, Cp, zrel, q, F, st, ones, stage + 1024 * ng * plane);
     };
     if constexpr (BL) {
         // every lane's B operand of each K group (and plane), 4 groups of loads in flight
@@ -768,7 +775,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
     // sweep is loaded during the one before (past the last super-tile: row block 0 again)
     PSet ops;
     const int32_t rows32 = (int32_t)rows;
-    p_load<BL, ConstW>(sweep(P.X, C, 0, TileMean, 0), 0, ng, q, rows32, ops);
+    p_load<BL, Direct>(sweep(P.X, C, 0, TileMean, 0), 0, ng, q, rows32, ops);
     for (int64_t st = 0; st < nsuper; ++st) {
         const int32_t kc = (int32_t)(64 * st) - F + lane;   // K <= N <= 2^30
         const bool cand = kc >= 0 && kc < (int32_t)K;
@@ -784,7 +791,7 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
         const int64_t stn = st + 1 < nsuper ? st + 1 : 0;
         const PSweep wa = sweep(P.X, C, st, TileMean, 0);
         if constexpr (ConstW) {
-            super_tile_prows<BL, true>(wa, sweep(P.X, C, stn, TileMean, 0), ng, q, rows32, ops,
+            super_tile_prows<BL, Direct>(wa, sweep(P.X, C, stn, TileMean, 0), ng, q, rows32, ops,
                                        ring0,
                                  ringA);
         } else {
