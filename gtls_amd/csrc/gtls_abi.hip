@@ -79,7 +79,8 @@ int hip_fail(gtls_ctx* ctx, hipError_t e, const char* where) {
 struct SearchLayout {
     int64_t uw0 = 0, tmpl2 = 0, tmpl1 = 0, tmplsq = 0, ss = 0, wconst = 0, flat = 0, counters = 0;
     int64_t tmpl_tail = 0;
-    int64_t tmplh = 0, tmplc = 0, xsc = 0, n_chunked = 0, n_zero = 0, tplen = 0, cplen = 0;
+    int64_t tmplh = 0, tmplc = 0, xsc = 0, upair = 0, n_chunked = 0, n_zero = 0, tplen = 0;
+    int64_t cplen = 0;
     int64_t slots = 0, slot_bytes = 0, o_key = 0, o_perm = 0, o_tmp_key = 0, o_tmp_id = 0;
     int64_t o_lists = 0, n_slots = 0;
     int64_t xbuf = 0, dbuf = 0, meta = 0, part = 0;
@@ -120,6 +121,8 @@ SearchLayout plan_search(const gtls_ctx* ctx, int mode, const gtls_batch* b) {
     off = align_up(off + 8 * L.cplen);
     L.xsc = off;
     off = align_up(off + 8 * (int64_t)b->n_curves);
+    L.upair = off;
+    off = align_up(off + (mode == GTLS_MODE_FP32C ? 4 * b->total_points : 0));
     L.ss = off;
     off = align_up(off + 8 * (b->total_widths > 0 ? b->total_widths : 1));
     L.wconst = off;
@@ -241,6 +244,10 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
                        st, b->tmpl, b->tmpl_off, b->widths, b->total_widths, ss);
     hipLaunchKernelGGL(chi2_flat_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
                        b->y, b->dy, flat, wconst, (float2*)(ws + L.xsc));
+    if (sizeof(Real) == 4)
+        hipLaunchKernelGGL(pair_points_kernel, dim3(b->n_curves), dim3(kBlock), 0, st, b->pt_off,
+                           (const float2*)uw0, (const float2*)(ws + L.xsc),
+                           (uint32_t*)(ws + L.upair));
     const int64_t ncount = (int64_t)kChunkCounters * L.n_chunks;
     hipLaunchKernelGGL(zero_u64s_kernel, dim3((unsigned)std::min<int64_t>(1024, (ncount + tpb - 1) / tpb)),
                        dim3(tpb), 0, st, counters, ncount);
@@ -267,6 +274,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.clo = (int32_t)(4 * L.cplen);
     p.zoff = L.n_chunked;
     p.xsc = (const float2*)(ws + L.xsc);
+    p.upair = (const uint32_t*)(ws + L.upair);
     // curves with a common weight need the prefix sums D only if some width of the batch
     // takes a path that reads them (the tiles compute their window means themselves)
     p.d_constw = 0;

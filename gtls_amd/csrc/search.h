@@ -71,6 +71,8 @@ struct SearchParams {
     int32_t tlo, clo;
     int64_t zoff;
     const float2* xsc;      // per curve: power-of-two scales of the A-sum and weight planes
+    const uint32_t* upair;  // float32: per point the (hi, lo) pair of u scaled by its curve's
+                            // A-sum plane scale (the samples of a common weight)
     int32_t d_constw;       // float32: curves with a common weight need the prefix sums D
                             // (some width takes a path that reads them)
     const double* ss;       // per width: sum of S^2 over the row
@@ -287,22 +289,23 @@ __device__ __forceinline__ void gather_scan_window(const float2* __restrict__ uw
 }
 
 // The samples of one sort window in float32 with a common weight when no running sums are
-// wanted: position o0 + j takes the (hi, lo) pair of sx u of sample sidx[j]. All of a
-// thread's gathers are in flight before its first store. All threads call it; no barrier.
-__device__ __forceinline__ void gather_window_pairs(const float2* __restrict__ uw, int64_t o0,
+// wanted: position o0 + j takes the (hi, lo) pair of sx u of sample sidx[j], prepared per
+// point in up (pair_points_kernel). All of a thread's gathers are in flight before its
+// first store. All threads call it; no barrier.
+__device__ __forceinline__ void gather_window_pairs(const uint32_t* __restrict__ up, int64_t o0,
                                                     int cnt, const int32_t* sidx, uint32_t* hb,
-                                                    float sx, int64_t n, int64_t wrap) {
-    float u[kWinPer];
+                                                    int64_t n, int64_t wrap) {
+    uint32_t u[kWinPer];
 #pragma unroll
     for (int e = 0; e < kWinPer; ++e) {
         const int j = threadIdx.x + e * kBlock;
-        u[e] = j < cnt ? uw[sidx[j]].x : 0.f;
+        u[e] = j < cnt ? up[sidx[j]] : 0u;
     }
 #pragma unroll
     for (int e = 0; e < kWinPer; ++e) {
         const int j = threadIdx.x + e * kBlock;
         if (j < cnt) {
-            const uint32_t v = split_pair(u[e] * sx);
+            const uint32_t v = u[e];
             __builtin_nontemporal_store(v, &hb[o0 + j]);
             // the wrap-around extension: position n + p repeats position p < wrap
             if (o0 + j < wrap) __builtin_nontemporal_store(v, &hb[n + o0 + j]);
@@ -398,8 +401,8 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(SearchParams<Real> p) {
                 s_idx, s_mids, s_wtab, s_wb, s_ws, s_wcur, [&](int wstart, int wcnt) {
                     if constexpr (sizeof(Real) == 4) {
                         if (constw && !D) {
-                            gather_window_pairs((const float2*)uw, wstart, wcnt, s_idx, sink.hb,
-                                                sink.sx, n, wrap_in_sink);
+                            gather_window_pairs(p.upair + pt0, wstart, wcnt, s_idx, sink.hb, n,
+                                                wrap_in_sink);
                             return;
                         }
                         if (constw) {
@@ -881,6 +884,19 @@ __global__ __launch_bounds__(kBlock) void chi2_flat_kernel(const int64_t* pt_off
         xsc[c] = differ ? make_float2(f16_scale(mv), f16_scale(mw))
                         : make_float2(f16_scale(mu), 1.f);
     }
+}
+
+// Per point the float16 (hi, lo) pair of u scaled by its curve's A-sum plane scale, as the
+// float32 search's common-weight samples take it; one workgroup per curve (after
+// chi2_flat_kernel, which sets the scales).
+__global__ __launch_bounds__(kBlock) void pair_points_kernel(const int64_t* pt_off,
+                                                             const float2* uw,
+                                                             const float2* xsc,
+                                                             uint32_t* upair) {
+    const int c = blockIdx.x;
+    const int64_t a = pt_off[c], b = pt_off[c + 1];
+    const float sx = xsc[c].x;
+    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) upair[i] = split_pair(uw[i].x * sx);
 }
 
 }  // namespace gtls
