@@ -445,7 +445,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     for (int b = tid; b < B; b += nt) hist[hslot(b)] = 0;
     if (tid < 4) counts[tid] = 0;
     __syncthreads();
-    constexpr int U = 8;   // samples in flight per thread
+    constexpr int U = 16;   // samples in flight per thread
     bool small = rP != rP;
     for (int64_t base = 0; base < n; base += (int64_t)U * nt) {
         double tv[U];
