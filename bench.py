@@ -228,7 +228,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--tuning", default=None, metavar="TOEP_Q,CHUNK_MB,WG_PER_CU",
+    ap.add_argument("--tuning", default=None, metavar="TOEP_Q,CHUNK_MB,WG_PER_CU[,QUAD_Q]",
                     help="apply this setting instead of the autotuner's choice")
     ap.add_argument("--no-full-check", action="store_true",
                     help="skip the float64 search of a whole batch (parity of every curve)")
@@ -296,23 +296,26 @@ def main():
     tuning = None
     if args.tuning:
         setting = tuple(int(x) for x in args.tuning.split(","))
-        if len(setting) != 3:
-            print("--tuning takes TOEP_Q,CHUNK_MB,WG_PER_CU", file=sys.stderr)
+        if len(setting) not in (3, 4):
+            print("--tuning takes TOEP_Q,CHUNK_MB,WG_PER_CU[,QUAD_Q]", file=sys.stderr)
             sys.exit(2)
+        if len(setting) == 3:
+            setting = setting + (tune.DEFAULT[3],)
         tune.set_tuning(0, *setting)
         tuning = {"choice": {"toep_q": setting[0], "chunk_mb": setting[1],
-                             "wg_per_cu": setting[2]}, "source": "--tuning"}
+                             "wg_per_cu": setting[2], "quad_q": setting[3]},
+                  "source": "--tuning"}
     elif not args.no_tune:
         ts = time.perf_counter()
         choice, times = tune.autotune(batches[0][1], skip, n_sample=32, repeats=1)
         tuning = {"choice": {"toep_q": choice[0], "chunk_mb": choice[1],
-                             "wg_per_cu": choice[2]},
-                  "candidates_ms": {"%d/%d/%d" % k: round(v, 2) for k, v in times.items()},
+                             "wg_per_cu": choice[2], "quad_q": choice[3]},
+                  "candidates_ms": {"%d/%d/%d/%d" % k: round(v, 2) for k, v in times.items()},
                   "seconds": round(time.perf_counter() - ts, 1),
                   "sample": "32 curves of batch 0 spread over its sizes, one timed call per "
                             "setting; coordinate search (workgroups/CU, chunk budget, "
-                            "Toeplitz bound in turn)"}
-        log("autotune: toep_q %d, chunk %d MiB, %d workgroups/CU (%.1f s)" % (
+                            "Toeplitz bound, operand layout threshold in turn)"}
+        log("autotune: toep_q %d, chunk %d MiB, %d workgroups/CU, quad_q %d (%.1f s)" % (
             choice + (tuning["seconds"],)))
 
     # one graph per batch, all over one workspace (replayed in stream order), each with

@@ -82,7 +82,7 @@ def _declare(lib):
         "gtls_transit_chi2": (ct.c_int, [P, I32, I64, P, P, P, P, P, P, P, P]),
         "gtls_set_timing": (ct.c_int, [P, I32]),
         "gtls_kernel_ms": (I64, [P, ct.POINTER(F64)]),
-        "gtls_set_tuning": (ct.c_int, [P, I32, I64, I32]),
+        "gtls_set_tuning": (ct.c_int, [P, I32, I64, I32, I32]),
         "gtls_set_graph_timing": (ct.c_int, [P, P, I64]),
         "gtls_transit_tile_points": (I64, []),
         "gtls_transit_chi2_grouped": (ct.c_int, [P, I32, I32, I64, I32, P, P, P, P, I64, P, P, P,

@@ -23,6 +23,7 @@ struct gtls_ctx {
     int slots_sort = 0;
     int use_mfma = 1;
     int toep_q = gtls::kToepQDefault;
+    int quad_q = gtls::kQuadQ;          // common-weight strides from this on: Quad operand loads
     int wg_per_cu = 0;                  // cap on resident search workgroups per CU (0: none)
     int64_t chunk_bytes = 0;            // budget of one chunk's gathered samples
     std::string err;
@@ -294,6 +295,7 @@ int launch_search(gtls_ctx* ctx, const gtls_batch* b, char* ws, const SearchLayo
     p.negative_time = b->negative_time;
     p.use_mfma = ctx->use_mfma;
     p.toep_q = ctx->toep_q;
+    p.quad_q = ctx->quad_q;
     p.max_points = b->max_points;
     p.max_width = b->max_width > 0 ? b->max_width : 1;
     p.kmax = (int32_t)L.kmax;
@@ -473,14 +475,17 @@ int gtls_set_stream(gtls_ctx* ctx, void* stream) {
 
 const char* gtls_last_error(const gtls_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb, int32_t wg_per_cu) {
+int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb, int32_t wg_per_cu,
+                    int32_t quad_q) {
     if (!ctx) return GTLS_E_SHAPE;
     if (toep_q < 0 || toep_q > kToepQ) return fail(ctx, GTLS_E_SHAPE, "toep_q out of [0, 24]");
     if (chunk_mb < 0) return fail(ctx, GTLS_E_SHAPE, "chunk_mb < 0");
     if (wg_per_cu < 0) return fail(ctx, GTLS_E_SHAPE, "wg_per_cu < 0");
+    if (quad_q < 0) return fail(ctx, GTLS_E_SHAPE, "quad_q < 0");
     ctx->wg_per_cu = wg_per_cu;
     ctx->toep_q = toep_q;
     if (chunk_mb > 0) ctx->chunk_bytes = chunk_mb << 20;
+    ctx->quad_q = quad_q > 0 ? quad_q : kQuadQ;
     return GTLS_OK;
 }
 

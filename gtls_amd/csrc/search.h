@@ -83,6 +83,8 @@ struct SearchParams {
     int32_t negative_time;
     int32_t use_mfma;     // float32: window sums on the matrix cores
     int32_t toep_q;       // float32: strides below this take the Toeplitz tiles (<= kToepQ)
+    int32_t quad_q;       // float32, common weight: diagonal-sum strides from this on load
+                          // their sample operands in the Quad layout (windows.h)
     int32_t max_points;   // every curve must have n <= max_points
     int32_t max_width;    // every searched width must be <= max_width
     int32_t kmax;         // items per period in the window stage (>= widths per period)
@@ -548,7 +550,7 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
             const int32_t zrel = (int32_t)(4 * (p.zoff - co));
             // template operands staged in LDS when they fit
             const int32_t ng = chunk_pad(q) / kKGroup;
-            if (constw && F == 8 && q >= kQuadQ) {
+            if (constw && F == 8 && q >= p.quad_q) {
                 if (ng <= kStageGroupsConstW)
                     w = mfma_width<true, true, true, 8, false>(hp, C, nullptr, zrel, D, a);
                 else

@@ -523,6 +523,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 //    32 h + 4 r8 + s: row 8 h + r8, slice s), and ds_bpermute (source lane 4 r + s) moves
 //    them to the operand layout, 4 KiB per K group and wavefront through the CU's LDS
 //    crossbar.
+// kQuadQ is the default of the tunable threshold (gtls_set_tuning).
 // Rows are 4 q bytes apart. Measured on the batched workload with a common weight: Direct
 // everywhere 6.4 % faster than Quad everywhere (with Quad the LDS was active 63 % of the
 // kernel's cycles); Quad from q = 96 on (where the sweeps are long and the operand moves

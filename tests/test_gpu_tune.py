@@ -20,9 +20,10 @@ def test_autotune_picks_a_candidate_and_keeps_results(gpu):
     try:
         choice, times = tune.autotune(db, n_sample=12, repeats=1)
         # every value of every axis was timed, at most one setting per value and axis
-        for axis, values in enumerate((tune.TOEP_Q, tune.CHUNK_MB, tune.WG_PER_CU)):
+        for axis, values in enumerate((tune.TOEP_Q, tune.CHUNK_MB, tune.WG_PER_CU, tune.QUAD_Q)):
             assert {s[axis] for s in times} == set(values)
-        assert len(times) <= len(tune.TOEP_Q) + len(tune.CHUNK_MB) + len(tune.WG_PER_CU)
+        assert len(times) <= (len(tune.TOEP_Q) + len(tune.CHUNK_MB) + len(tune.WG_PER_CU)
+                              + len(tune.QUAD_Q))
         assert all(np.isfinite(v) and v > 0 for v in times.values())
         assert choice in times and times[choice] == min(times.values())
         for q, wg in ((q, wg) for q in tune.TOEP_Q for wg in tune.WG_PER_CU):
@@ -32,5 +33,13 @@ def test_autotune_picks_a_candidate_and_keeps_results(gpu):
             ok = torch.isfinite(c0)
             rel = ((c - c0).abs() / c0.abs())[ok]
             assert float(rel.max()) < 2e-6, (q, wg)
+        # the operand load layout changes nothing in the results
+        tune.set_tuning(0, 8, 0, 0, 96)
+        c1 = db.search("fp32c", 8)[0].clone()
+        for quad in (1, 64, 1 << 20):
+            tune.set_tuning(0, 8, 0, 0, quad)
+            c = db.search("fp32c", 8)[0]
+            torch.cuda.synchronize()
+            assert torch.equal(torch.nan_to_num(c, nan=-1.0), torch.nan_to_num(c1, nan=-1.0)), quad
     finally:
         tune.set_tuning(0, *tune.DEFAULT)
