@@ -331,9 +331,11 @@ class Refiner:
     with sP = D0 P0 / (4 span) (a quarter duration of drift over the baseline) and
     sT = D0 / 4. With fused (the default) the loss and its gradient come from one kernel
     pass per step (gtls_transit_chi2); otherwise from the flux kernels and torch's
-    segmented sum."""
+    segmented sum. With capturable, the optimizer keeps its step counts on the device, so
+    that capture() can record a whole step in a HIP graph."""
 
-    def __init__(self, mb: ModelBatch, theta0, spans, mode="fp32c", lr=0.05, fused=True):
+    def __init__(self, mb: ModelBatch, theta0, spans, mode="fp32c", lr=0.05, fused=True,
+                 capturable=False):
         torch = _torch()
         self.mb = mb
         self.mode = mode
@@ -347,7 +349,8 @@ class Refiner:
         z[:, 4] = _logit(th[:, 4])
         z[:, 5] = _logit(th[:, 5])
         self.z = torch.tensor(z, dtype=torch.float64, device=mb.dev, requires_grad=True)
-        self.opt = torch.optim.Adam([self.z], lr=lr, fused=True)
+        self.opt = torch.optim.Adam([self.z], lr=lr, fused=True, capturable=capturable)
+        self.capturable = capturable
 
     def theta(self):
         torch = _torch()
@@ -372,6 +375,36 @@ class Refiner:
         per_model.sum().backward()
         self.opt.step()
         return per_model.detach()
+
+    def capture(self, warmup=2):
+        """A HIP graph (torch.cuda.CUDAGraph) of one step: replay() runs the parameter
+        transform, the fused loss and its gradient, the backward and the Adam update with
+        no host work; the graph's `loss` tensor holds the per-model chi2 of the parameters
+        the replayed step started from. Needs fused and capturable. The `warmup` steps run
+        first, as ordinary steps on the capture stream (they create that stream's library
+        context and the optimizer state before the capture, so nothing is allocated
+        outside torch's graph pool while recording)."""
+        torch = _torch()
+        if not (self.fused and self.capturable):
+            raise ValueError("capture() needs a Refiner built with fused=True, capturable=True")
+        if _TIMING is not None:
+            raise RuntimeError("switch kernel timing off (set_timing(False)) before capture()")
+        dev = self.mb.dev
+        s = torch.cuda.Stream(dev)
+        s.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(s):
+            for _ in range(max(1, warmup)):
+                self.step()
+        torch.cuda.current_stream(dev).wait_stream(s)
+        torch.cuda.synchronize(dev)
+        g = torch.cuda.CUDAGraph()
+        self.opt.zero_grad(set_to_none=True)
+        with torch.cuda.graph(g, stream=s):
+            loss = self.loss()
+            loss.sum().backward()
+            self.opt.step()
+        g.loss = loss
+        return g
 
     def run(self, steps=200, progress=None):
         """Adam steps; returns the per-model chi2 before the first and after the last.

@@ -253,3 +253,32 @@ def test_refinement_recovers_injected_transits(gpu, fused):
         assert abs(dt) < 0.1 * D
         assert got[m, 2] == pytest.approx(D, rel=0.15)
         assert got[m, 3] == pytest.approx(depth, rel=0.15)
+
+
+@pytest.mark.gpu
+def test_captured_step_replays_match_eager_steps(gpu):
+    """A HIP graph of one refinement step (Refiner.capture) replayed 6 times gives the
+    eager steps' parameters and losses bit for bit; between replays the process runs eager
+    work of its own and synchronises (every replay is checked)."""
+    mb, th, curves, mc, _ = model_batch(seed=4, n=6000)
+    spans = np.array([np.ptp(curves[c][0]) for c in mc])
+    a = R.Refiner(mb, th, spans, mode="fp32c", capturable=True)
+    want_z, want_loss = [], []
+    for i in range(8):
+        loss = a.step()
+        torch.cuda.synchronize()
+        want_z.append(a.z.detach().clone())
+        want_loss.append(loss.clone())
+    b = R.Refiner(mb, th, spans, mode="fp32c", capturable=True)
+    g = b.capture(warmup=2)
+    torch.cuda.synchronize()
+    assert torch.equal(b.z.detach(), want_z[1])
+    for i in range(2, 8):
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(g.loss, want_loss[i]), i
+        assert torch.equal(b.z.detach(), want_z[i]), i
+        # eager work between replays
+        other = R.chi2_fused(b.theta().detach(), mb, "fp32c")
+        torch.cuda.synchronize()
+        assert torch.all(torch.isfinite(other))
