@@ -838,8 +838,10 @@ int gtls_transit_chi2_grouped(gtls_ctx* ctx, int32_t mode, int32_t n_curves, int
     if (!theta || !model_off || !pt_off || !tile_off || !t || !y || !w || !chi2)
         return fail(ctx, GTLS_E_SHAPE, "null transit model argument");
     const int sums = grad_theta ? 7 : 1;
-    if (n_tiles > 0 && (!part || part_len < n_tiles * (int64_t)max_models * sums))
-        return fail(ctx, GTLS_E_WORKSPACE, "tile sums need n_tiles * max_models * (7 or 1) doubles");
+    if (n_tiles > 0 && (!part || part_len < n_tiles * ((int64_t)max_models * sums + 1)))
+        return fail(ctx, GTLS_E_WORKSPACE,
+                    "tile sums need n_tiles * (max_models * (7 or 1) + 1) doubles");
+    double* flat = part + n_tiles * (int64_t)max_models * sums;
     DeviceGuard guard(ctx->device);
     const dim3 tiles((unsigned)n_tiles), block(kChi2Threads);
     if (n_tiles > 0) {
@@ -847,32 +849,32 @@ int gtls_transit_chi2_grouped(gtls_ctx* ctx, int32_t mode, int32_t n_curves, int
             if (grad_theta)
                 hipLaunchKernelGGL((transit_chi2_tile_kernel<double, true>), tiles, block, 0,
                                    ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
-                                   max_models, t, y, w, part);
+                                   max_models, t, y, w, part, flat);
             else
                 hipLaunchKernelGGL((transit_chi2_tile_kernel<double, false>), tiles, block, 0,
                                    ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
-                                   max_models, t, y, w, part);
+                                   max_models, t, y, w, part, flat);
         } else {
             if (grad_theta)
                 hipLaunchKernelGGL((transit_chi2_tile_kernel<float, true>), tiles, block, 0,
                                    ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
-                                   max_models, t, y, w, part);
+                                   max_models, t, y, w, part, flat);
             else
                 hipLaunchKernelGGL((transit_chi2_tile_kernel<float, false>), tiles, block, 0,
                                    ctx->stream, theta, model_off, pt_off, tile_off, n_curves,
-                                   max_models, t, y, w, part);
+                                   max_models, t, y, w, part, flat);
         }
     }
     const int64_t nsum = n_models * sums;
     const dim3 sgrid((unsigned)((nsum + 255) / 256)), sblock(256);
     if (grad_theta)
         hipLaunchKernelGGL(transit_chi2_sum_kernel<true>, sgrid, sblock, 0, ctx->stream,
-                           model_off, tile_off, n_curves, max_models, n_models, part, chi2,
-                           grad_theta);
+                           model_off, tile_off, n_curves, max_models, n_models, part, flat,
+                           chi2, grad_theta);
     else
         hipLaunchKernelGGL(transit_chi2_sum_kernel<false>, sgrid, sblock, 0, ctx->stream,
-                           model_off, tile_off, n_curves, max_models, n_models, part, chi2,
-                           grad_theta);
+                           model_off, tile_off, n_curves, max_models, n_models, part, flat,
+                           chi2, grad_theta);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "grouped transit chi2 launch");
     return GTLS_OK;

@@ -244,14 +244,24 @@ __global__ __launch_bounds__(chi2_block<Real>()) void transit_chi2_kernel(const 
 // consecutive points, tiles [tile_off[c], tile_off[c + 1]) of the launch; the models of
 // curve c are [model_off[c], model_off[c + 1]) (at most max_models). Workgroup i takes tile
 // i: its kChi2PerThread points per thread stay in registers while every model of the curve
-// is evaluated on them, and the tile's sums go to
+// is evaluated on them. A point whose fold lies more than kNearHalf durations from the
+// model's transit (|tau| > kNearHalf D, float64) is outside the model (its kappa s is below
+// -kLogCut: flux 1, zero derivatives) and is skipped; its share of chi2, w (y - 1)^2, is
+// that of the flat model, which the tile sums once for all models:
+//     flat[i] = sum_tile w (y - 1)^2,
+//     chi2_m = sum_tiles flat + sum_near w ((y - f)^2 - (y - 1)^2)
+//            = sum_tiles flat + sum_near w (1 - f) ((y - f) + (y - 1)).
+// The tile's model sums go to
 //     part[(i * max_models + j) * S + k]      (model m = model_off[c] + j, k < S),
-// S = 7 with Grad (chi2, then the six gradient components), else 1. Each point is read
-// once per call, whatever the number of models. transit_chi2_sum_kernel then adds every
-// model's tiles in tile order, so the result does not depend on the launch.
+// S = 7 with Grad (the chi2 difference, then the six gradient components), else 1. Each
+// point is read once per call, whatever the number of models. transit_chi2_sum_kernel then
+// adds every model's tiles in tile order, so the result does not depend on the launch.
 constexpr int kChi2Threads = 256;
 constexpr int kChi2PerThread = 8;
 constexpr int kChi2Tile = kChi2Threads * kChi2PerThread;
+// |x| = 2 |tau| / D above sqrt(1 + kLogCut / kappa) = 1.3919 is outside the model; the
+// float64 pre-test keeps a margin over the Real test of transit_point
+constexpr double kNearHalf = 0.71;
 
 // Last index c in [0, n) with off[c] <= i, for off ascending with off[0] <= i < off[n].
 __device__ __forceinline__ int owner_of(const int64_t* off, int n, int64_t i) {
@@ -268,9 +278,10 @@ __global__ __launch_bounds__(kChi2Threads) void transit_chi2_tile_kernel(
     const double* __restrict__ theta, const int64_t* __restrict__ model_off,
     const int64_t* __restrict__ pt_off, const int64_t* __restrict__ tile_off, int32_t n_curves,
     int32_t max_models, const double* __restrict__ t, const double* __restrict__ y,
-    const double* __restrict__ w, double* __restrict__ part) {
+    const double* __restrict__ w, double* __restrict__ part, double* __restrict__ flat) {
     constexpr int kSums = Grad ? 7 : 1;
     __shared__ double red[2][kSums][kChi2Threads / kWave];
+    __shared__ double red_flat[kChi2Threads / kWave];
     const int64_t i = blockIdx.x;
     const int c = owner_of(tile_off, n_curves, i);
     const int64_t a = pt_off[c] + (i - tile_off[c]) * kChi2Tile;
@@ -289,20 +300,38 @@ __global__ __launch_bounds__(kChi2Threads) void transit_chi2_tile_kernel(
     }
     const int64_t m0 = model_off[c], m1 = model_off[c + 1];
     const int lane = tid & (kWave - 1), wave = tid / kWave;
+    // the flat model's chi2 over the tile (the barrier of the first model orders it)
+    {
+        double fs = 0.0;
+#pragma unroll
+        for (int u = 0; u < kChi2PerThread; ++u) {
+            const double r0 = yv[u] - 1.0;
+            fs = fma(wv[u] * r0, r0, fs);
+        }
+        fs = wave_sum(fs);
+        if (lane == 0) red_flat[wave] = fs;
+    }
     for (int64_t m = m0; m < m1; ++m) {
         double th[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) th[k] = theta[6 * m + k];
         const double inv_p = 1.0 / th[0];
+        const double near = kNearHalf * th[2];
         double acc[kSums];
 #pragma unroll
         for (int k = 0; k < kSums; ++k) acc[k] = 0.0;
+        bool any = false;
 #pragma unroll
         for (int u = 0; u < kChi2PerThread; ++u) {
+            const double e = tv[u] - th[1];
+            const double tau = e - rint(e * inv_p) * th[0];
+            if (!(fabs(tau) <= near)) continue;   // outside the model: the flat share
+            any = true;
             const TransitPoint<Real> pt = transit_point<Real, Grad>(tv[u], th, inv_p);
-            const double r = yv[u] - (double)pt.f;
+            const double f = (double)pt.f;
+            const double r = yv[u] - f;
             const double wr = wv[u] * r;
-            acc[0] = fma(wr, r, acc[0]);
+            acc[0] = fma(wv[u] * (1.0 - f), r + (yv[u] - 1.0), acc[0]);
             if constexpr (Grad) {
                 const double g = -2.0 * wr;
                 acc[1] = fma(g, (double)pt.dP, acc[1]);
@@ -315,12 +344,17 @@ __global__ __launch_bounds__(kChi2Threads) void transit_chi2_tile_kernel(
         }
         // workgroup sums; the two halves of red alternate between models, so one barrier
         // per model orders every write of red before its reads and after the reads of the
-        // model two back
+        // model two back. A wavefront with no point near a transit of the model has only
+        // zero sums (no term was added) and skips its reductions.
         const int h = (int)((m - m0) & 1);
+        if (__ballot(any) != 0ULL) {
 #pragma unroll
-        for (int k = 0; k < kSums; ++k) {
-            const double v = wave_sum(acc[k]);
-            if (lane == 0) red[h][k][wave] = v;
+            for (int k = 0; k < kSums; ++k) {
+                const double v = wave_sum(acc[k]);
+                if (lane == 0) red[h][k][wave] = v;
+            }
+        } else if (lane < kSums) {
+            red[h][lane][wave] = 0.0;
         }
         __syncthreads();
         if (tid < kSums) {
@@ -330,15 +364,23 @@ __global__ __launch_bounds__(kChi2Threads) void transit_chi2_tile_kernel(
             part[(i * max_models + (m - m0)) * kSums + tid] = sum;
         }
     }
+    if (m1 == m0) __syncthreads();
+    if (tid == kSums) {
+        double sum = 0.0;
+#pragma unroll
+        for (int v = 0; v < kChi2Threads / kWave; ++v) sum += red_flat[v];
+        flat[i] = sum;
+    }
 }
 
-// chi2[m] (and grad_theta[6 m + k]) as the sum of model m's tile sums in tile order; one
-// thread per (model, sum).
+// chi2[m] (and grad_theta[6 m + k]) as the sum of model m's tile sums in tile order (for
+// chi2, then the flat sums of its curve's tiles in tile order); one thread per (model, sum).
 template <bool Grad>
 __global__ void transit_chi2_sum_kernel(const int64_t* __restrict__ model_off,
                                         const int64_t* __restrict__ tile_off, int32_t n_curves,
                                         int32_t max_models, int64_t n_models,
-                                        const double* __restrict__ part, double* __restrict__ chi2,
+                                        const double* __restrict__ part,
+                                        const double* __restrict__ flat, double* __restrict__ chi2,
                                         double* __restrict__ grad_theta) {
     constexpr int kSums = Grad ? 7 : 1;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -350,6 +392,11 @@ __global__ void transit_chi2_sum_kernel(const int64_t* __restrict__ model_off,
     double sum = 0.0;
     for (int64_t i = tile_off[c]; i < tile_off[c + 1]; ++i)
         sum += part[(i * max_models + j) * kSums + k];
+    if (k == 0) {
+        double fs = 0.0;
+        for (int64_t i = tile_off[c]; i < tile_off[c + 1]; ++i) fs += flat[i];
+        sum += fs;
+    }
     if (k == 0) chi2[m] = sum;
     else grad_theta[6 * m + k - 1] = sum;
 }

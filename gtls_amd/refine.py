@@ -84,7 +84,7 @@ class ModelBatch:
     n_curves: int = 0
     n_tiles: int = 0
     max_models: int = 0
-    part: object = None        # float64 [n_tiles * max_models * 7]
+    part: object = None        # float64 [n_tiles * (max_models * 7 + 1)]
 
     @classmethod
     def build(cls, curves, model_curve, device=0):
@@ -126,7 +126,7 @@ class ModelBatch:
                    None if grouped else torch.as_tensor(np.argsort(order), **i64),
                    torch.as_tensor(model_off, **i64), torch.as_tensor(tile_off, **i64),
                    n_curves, n_tiles, max_models,
-                   torch.empty(max(1, n_tiles * max_models * 7), **f64))
+                   torch.empty(max(1, n_tiles * (max_models * 7 + 1)), **f64))
 
     @property
     def slots(self):

@@ -240,9 +240,12 @@ int gtls_transit_chi2(gtls_ctx* ctx, int32_t mode, int64_t n_models, const doubl
  * gtls_transit_tile_points() consecutive points, tiles [tile_off[c], tile_off[c + 1]) with
  * tile_off[c + 1] - tile_off[c] = ceil(N_c / tile points) and n_tiles = tile_off[n_curves];
  * each tile is read once and every model of its curve is evaluated on it, so a call reads
- * each point once whatever the number of models. part is caller scratch of part_len >=
- * n_tiles * max_models * (grad_theta ? 7 : 1) doubles for the per-tile sums, which are
- * added in tile order (results do not depend on the launch). Outputs as gtls_transit_chi2.
+ * each point once whatever the number of models; a point farther than 0.71 durations from
+ * a model's transit (outside the model: flux 1) adds the flat model's w (y - 1)^2, summed
+ * once per tile for all models. part is caller scratch of part_len >=
+ * n_tiles * (max_models * (grad_theta ? 7 : 1) + 1) doubles for the per-tile sums, which
+ * are added in tile order (results do not depend on the launch). Outputs as
+ * gtls_transit_chi2 (the same sums, in another order).
  */
 int64_t gtls_transit_tile_points(void);
 int gtls_transit_chi2_grouped(gtls_ctx* ctx, int32_t mode, int32_t n_curves, int64_t n_models,
