@@ -90,10 +90,12 @@ __device__ __forceinline__ void sort8(double* k, int32_t* id) {
 #undef GTLS_CS
 }
 
-// Batcher's odd-even merge network for 16 keys (63 compare-exchanges).
-template <bool PhaseOnly = false>
+// Batcher's odd-even merge network for 16 keys (63 compare-exchanges); with N < 16 the
+// network for the first N inputs, the others +inf: comparators that touch an input >= N
+// are dropped (the +inf keys start at the top and only ever move up), 42 for N = 12.
+template <bool PhaseOnly = false, int N = 16>
 __device__ __forceinline__ void sort16(double* k, int32_t* id) {
-#define GTLS_CS(a, b) cmp_swap<PhaseOnly>(k[a], id[a], k[b], id[b])
+#define GTLS_CS(a, b) if constexpr ((b) < N) cmp_swap<PhaseOnly>(k[a], id[a], k[b], id[b])
     GTLS_CS(0, 1); GTLS_CS(2, 3); GTLS_CS(0, 2); GTLS_CS(1, 3); GTLS_CS(1, 2);
     GTLS_CS(4, 5); GTLS_CS(6, 7); GTLS_CS(4, 6); GTLS_CS(5, 7); GTLS_CS(5, 6);
     GTLS_CS(0, 4); GTLS_CS(2, 6); GTLS_CS(2, 4); GTLS_CS(1, 5); GTLS_CS(3, 7);
@@ -125,13 +127,13 @@ __device__ __forceinline__ void sort_bucket_regs(double* __restrict__ key, int32
     // on the phases alone, then again on the full key only when two of the bucket's
     // phases are equal (their order is then decided by the index)
     if constexpr (M == 8) sort8<true>(k, id);
-    else sort16<true>(k, id);
+    else sort16<true, M>(k, id);
     bool tie = false;
 #pragma unroll
     for (int e = 0; e + 1 < M; ++e) tie |= (e + 1 < cnt) && k[e] == k[e + 1];
     if (tie) {
         if constexpr (M == 8) sort8(k, id);
-        else sort16(k, id);
+        else sort16<false, M>(k, id);
     }
 #pragma unroll
     for (int e = 0; e < M; ++e) {
@@ -619,11 +621,13 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         for (int b = bfirst + tid; b < bnext; b += nt) {
             const int blo = (b > bfirst ? hist[hslot(b - 1)] : wstart) - wstart;
             const int cnt = hist[hslot(b)] - wstart - blo;
-            // one network per pass of the wavefront: the 16-key one for all of its lanes
-            // when any of them holds 9-16 keys (lanes would otherwise run both networks)
-            const bool wide = __ballot(cnt > 8 && cnt <= 16) != 0;
+            // one network per pass of the wavefront, the smallest that takes its largest
+            // bucket of up to 16 keys (lanes would otherwise run several networks)
+            const bool w16 = __ballot(cnt > 12 && cnt <= 16) != 0;
+            const bool w12 = __ballot(cnt > 8 && cnt <= 12) != 0;
             if (cnt > 1 && cnt <= 16) {
-                if (wide) sort_bucket_regs<16>(sphi, sidx, blo, cnt);
+                if (w16) sort_bucket_regs<16>(sphi, sidx, blo, cnt);
+                else if (w12) sort_bucket_regs<12>(sphi, sidx, blo, cnt);
                 else sort_bucket_regs<8>(sphi, sidx, blo, cnt);
             } else if (cnt > 16) {
                 const int e = atomicAdd(&counts[1], 1);
