@@ -567,7 +567,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     // again (the times are shared by the curve's periods and stay in cache; a stored bucket
     // would be per-period scratch traffic)
     {
-        constexpr int UW = 20;   // samples in flight per thread (24 or more spill into AGPRs)
+        constexpr int UW = 16;   // samples in flight per thread (20 or more spill into AGPRs)
         for (int64_t base = 0; base < n; base += (int64_t)UW * nt) {
             int bv[UW];
             double tq[UW];
@@ -581,10 +581,28 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
                 const int64_t j = base + (int64_t)u * nt + tid;
                 bv[u] = j < n ? phase_bucket(fold_phase_r(tq[u], P, rP), B, neg) : -1;
             }
+            // consecutive lanes hold consecutive samples, whose windows rise within a cycle
+            // of the period: each run of lanes with one window takes its slots with one
+            // atomic of its first lane
+            const int lane = lane_id();
 #pragma unroll
             for (int u = 0; u < UW; ++u) {
                 const int64_t j = base + (int64_t)u * nt + tid;
-                if (bv[u] >= 0) binned[atomicAdd(&wcur[wtab[bv[u]]], 1)] = (int32_t)j;
+                const int win = bv[u] >= 0 ? (int)wtab[bv[u]] : -1;
+                const int prev = __shfl_up(win, 1, kWave);
+                const bool head = lane == 0 || prev != win;
+                const unsigned long long heads = __ballot(head);
+                int first = 0;
+                if (head && win >= 0) {
+                    const unsigned long long above = lane == kWave - 1 ? 0ULL : heads >> (lane + 1);
+                    const int len = above ? __ffsll((long long)above) : kWave - lane;
+                    first = atomicAdd(&wcur[win], len);
+                }
+                const unsigned long long upto =
+                    heads & (lane == kWave - 1 ? ~0ULL : ((2ULL << lane) - 1ULL));
+                const int hl = 63 - __clzll((long long)upto);
+                const int hfirst = __shfl(first, hl, kWave);
+                if (win >= 0) binned[hfirst + (lane - hl)] = (int32_t)j;
             }
         }
     }
