@@ -241,18 +241,22 @@ class DeviceBatch:
         with the graph (attribute `workspace`), so no later search or capture can free or
         share the memory its kernels write; `ws` passes one instead (the caller then keeps
         it alive and must not replay graphs that share it concurrently). The capture runs
-        with a library context of its own (attribute `gtls_ctx`); with `timing`, its
+        with a library context of its own (attribute `gtls_ctx`), given the tuning last
+        applied (tune.set_tuning) to the calling stream's context; with `timing`, its
         per-chunk kernel events are captured too and graph_kernel_ms(g) reads the figures
         of the last replay (wall-clock stamps written by one-thread kernels between the
         chunks' kernels, gtls_set_graph_timing). Every kernel of the call is captured (no
         memset nodes)."""
         import torch
-        if ws is None:
-            ws = torch.empty(self.workspace_bytes(mode, skip_point, depth_min),
-                             dtype=torch.uint8, device=self.dev)
+        from . import tune
+        setting = tune.applied(self.dev.index)
         s = torch.cuda.Stream(self.dev)
         s.wait_stream(torch.cuda.current_stream(self.dev))
         ctx = _ffi.Context(self.dev.index, s.cuda_stream)
+        tune.apply_to(ctx.handle, setting)
+        if ws is None:
+            ws = torch.empty(self.workspace_bytes(mode, skip_point, depth_min, ctx=ctx),
+                             dtype=torch.uint8, device=self.dev)
         stamps = None
         if timing:
             stamps = torch.zeros(4 * self.MAX_TIMED_CHUNKS, dtype=torch.int64, device=self.dev)

@@ -29,14 +29,32 @@ WG_PER_CU = (1, 2)
 QUAD_Q = (64, 96, 128, 1 << 20)
 DEFAULT = (8, 64 << 10, 0, 96)
 
+# the last setting applied through set_tuning, per (device, stream) context
+_APPLIED = {}
+
 
 def set_tuning(device=0, toep_q=8, chunk_mb=0, wg_per_cu=0, quad_q=0):
     """Apply a setting to the context of `device`'s current torch stream (quad_q 0: the
     library's default threshold)."""
     dev = _device(device)
     ctx = _ctx(dev)
-    _ffi.check(ctx.handle, _ffi.load().gtls_set_tuning(ctx.handle, int(toep_q), int(chunk_mb),
-                                                       int(wg_per_cu), int(quad_q)))
+    setting = (int(toep_q), int(chunk_mb), int(wg_per_cu), int(quad_q))
+    _ffi.check(ctx.handle, _ffi.load().gtls_set_tuning(ctx.handle, *setting))
+    _APPLIED[(ctx.device, ctx.stream)] = setting
+
+
+def applied(device=0):
+    """The setting last applied to the context of `device`'s current torch stream, or None
+    (the library's defaults)."""
+    dev = _device(device)
+    ctx = _ctx(dev)
+    return _APPLIED.get((ctx.device, ctx.stream))
+
+
+def apply_to(handle, setting):
+    """Apply a setting (from applied()) to another library context."""
+    if setting is not None:
+        _ffi.check(handle, _ffi.load().gtls_set_tuning(handle, *setting))
 
 
 def sample_batch(db, n=32):

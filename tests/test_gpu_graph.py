@@ -84,3 +84,29 @@ def test_graph_kernel_timing_reads_each_replay(gpu):
         assert n == n_eager > 0, (n, n_eager)
         assert pm > 0 and wm > 0 and rm >= 0, (pm, wm, rm)
         assert pm + wm + rm <= e0.elapsed_time(e1) * 1.05 + 0.05, r
+
+
+def test_capture_keeps_the_tuning_of_the_calling_stream(gpu):
+    """A graph captured after tune.set_tuning runs with that setting (its own library
+    context inherits it): the replay equals the eager search under the same setting bit
+    for bit, and that setting's results differ from the default one's (another tile form
+    for strides 8-23), so a capture on the defaults would show."""
+    from gtls_amd import tune
+    curves = synthetic.batch(12, seed=11, n_min=2000, n_max=6000)
+    db = B.DeviceBatch(B.BatchPlan(B.build_plans(curves, {})), 0)
+    try:
+        tune.set_tuning(0, *tune.DEFAULT)
+        default = db.search("fp32c", 8)[0].clone()
+        tune.set_tuning(0, 24, 0, 0, 1)
+        want = [x.clone() for x in db.search("fp32c", 8)]
+        torch.cuda.synchronize()
+        assert not np.array_equal(default.cpu().numpy(), want[0].cpu().numpy(), equal_nan=True)
+        g = db.capture("fp32c", 8)
+        db.chi2.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(db.chi2.cpu().numpy(), want[0].cpu().numpy(), equal_nan=True)
+        assert torch.equal(db.best_k, want[1]) and torch.equal(db.best_s, want[2])
+    finally:
+        tune.set_tuning(0, *tune.DEFAULT)
