@@ -158,6 +158,20 @@ def test_tiny_time_takes_the_dividing_form(gpu, mode):
     assert_parity(plan, mode, 8, plan.periods[sel], plan.k_lo[sel], plan.k_hi[sel])
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32c"])
+def test_times_before_zero_match_oracle(gpu, mode):
+    """Times on both sides of zero (reachable through the C ABI and make_plan; the
+    reference's input cleaning drops them): folds with phases in (-1, 1), the bucket map
+    of negative times, against the oracle's fold of the same times."""
+    t, y = small_curve(1500, 9)
+    t = t - 1000.5   # about -0.5 .. 1.6 d
+    dy = np.full(t.size, float(np.std(y)))
+    plan = S.make_plan(t, y, dy, validate_args({}))
+    assert plan.t.min() < 0 < plan.t.max()
+    sel = np.arange(0, len(plan.periods), 5)
+    assert_parity(plan, mode, 8, plan.periods[sel], plan.k_lo[sel], plan.k_hi[sel])
+
+
 def test_edge_cases(gpu):
     t, y = small_curve(400, 3)
     plan = plan_for(t, y)
