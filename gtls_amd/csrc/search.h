@@ -570,6 +570,11 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
                     w = mfma_width<true, false, true>(hp, C, nullptr, zrel, D, a);
                 else
                     w = mfma_width<true, false, false>(hp, C, nullptr, zrel, D, a);
+            } else if (q < kQuadQWeighted) {
+                if (ng <= kStageGroupsWeighted)
+                    w = mfma_width<false, false, true, 0, true>(hp, C, C + p.clo, zrel, D, a);
+                else
+                    w = mfma_width<false, false, false, 0, true>(hp, C, C + p.clo, zrel, D, a);
             } else if (ng <= kStageGroupsWeighted) {
                 w = mfma_width<false, false, true>(hp, C, C + p.clo, zrel, D, a);
             } else {

@@ -512,13 +512,14 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 // ---------------------------------------------------------------------------------------
 // Diagonal-sum sweeps over K-group pairs: per tile, the sample operands of two K groups are
 // loaded together and run through their MFMAs as one step. Two load layouts (Direct):
-//  * Direct (common weight, strides below kQuadQ): each lane loads its operand in the MFMA
-//    layout itself (row l % 16 of the tile, K slice l / 16: 16 bytes), so one load touches
-//    16 rows of 64 bytes, the 4 lanes of a row 16 apart.
-//  * Quad (per-sample weights; common weight from stride kQuadQ on): a load covers 8 rows x
-//    128 bytes (two K groups): lane l reads row 8 h + (l / 4) % 8 of the tile (h = the
-//    tile's load 0 or 1), K group 2 p + l / 32, slice l % 4 (16 bytes), so most 128-byte
-//    lines are fetched once. v_permlane32_swap then gathers the first K group of all 16
+//  * Direct (common weight, strides below kQuadQ; per-sample weights, strides below
+//    kQuadQWeighted): each lane loads its operand in the MFMA layout itself (row l % 16 of
+//    the tile, K slice l / 16: 16 bytes), so one load touches 16 rows of 64 bytes, the 4
+//    lanes of a row 16 apart.
+//  * Quad (common weight from stride kQuadQ on, per-sample weights from kQuadQWeighted on):
+//    a load covers 8 rows x 128 bytes (two K groups): lane l reads row 8 h + (l / 4) % 8 of
+//    the tile (h = the tile's load 0 or 1), K group 2 p + l / 32, slice l % 4 (16 bytes), so
+//    most 128-byte lines are fetched once. v_permlane32_swap then gathers the first K group of all 16
 //    rows into the load-0 register and the second into the load-1 register (lanes
 //    32 h + 4 r8 + s: row 8 h + r8, slice s), and ds_bpermute (source lane 4 r + s) moves
 //    them to the operand layout, 4 KiB per K group and wavefront through the CU's LDS
@@ -528,9 +529,11 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 // everywhere 6.4 % faster than Quad everywhere (with Quad the LDS was active 63 % of the
 // kernel's cycles); Quad from q = 96 on (where the sweeps are long and the operand moves
 // few per candidate) another 6.6 % off the kernel's time on batch 0 (32: 2.1 %, 64: 4.7 %,
-// 128: 6.4 %, 192: 5.5 %, 256: 4.5 %). With per-sample weights Quad everywhere (Direct
-// 7.7 % slower).
+// 128: 6.4 %, 192: 5.5 %, 256: 4.5 %). With per-sample weights, Direct everywhere is 7.7 %
+// slower than Quad everywhere, but Direct below stride 24 is 4.7 % faster on 128 weighted
+// curves (bound 16: 5.3 %, 32: 4.8 %, 40: 3.3 %, 64: 2.3 %).
 constexpr int kQuadQ = 96;
+constexpr int kQuadQWeighted = 24;
 // ---------------------------------------------------------------------------------------
 struct PSweep {
     const char* X;       // sample pair plane (wave-uniform)
@@ -797,8 +800,8 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
                                  ringA);
         } else {
             const PSweep wb = sweep(P.W, C2, st, false, 1);
-            super_tile_prows<BL, false>(wa, wb, ng, q, rows32, ops, ring0, ringA);
-            super_tile_prows<BL, false>(wb, sweep(P.X, C, stn, false, 0), ng, q, rows32, ops,
+            super_tile_prows<BL, Direct>(wa, wb, ng, q, rows32, ops, ring0, ringA);
+            super_tile_prows<BL, Direct>(wb, sweep(P.X, C, stn, false, 0), ng, q, rows32, ops,
                                         ring0,
                                  ringB);
         }
