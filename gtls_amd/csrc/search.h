@@ -571,10 +571,10 @@ __device__ __forceinline__ ArgMin search_width(const SearchParams<Real>& p, int6
                 else
                     w = mfma_width<true, false, false>(hp, C, nullptr, zrel, D, a);
             } else if (q < kQuadQWeighted) {
-                if (ng <= kStageGroupsWeighted)
-                    w = mfma_width<false, false, true, 0, true>(hp, C, C + p.clo, zrel, D, a);
-                else
-                    w = mfma_width<false, false, false, 0, true>(hp, C, C + p.clo, zrel, D, a);
+                // strides this short fit the staged template operands
+                static_assert(chunk_pad(kQuadQWeighted - 1) / kKGroup <= kStageGroupsWeighted,
+                              "Direct weighted strides must fit the staged template");
+                w = mfma_width<false, false, true, 0, true>(hp, C, C + p.clo, zrel, D, a);
             } else if (ng <= kStageGroupsWeighted) {
                 w = mfma_width<false, false, true>(hp, C, C + p.clo, zrel, D, a);
             } else {

@@ -358,7 +358,7 @@ constexpr int kOnesCol = 14;     // tile column of the ones chunk (F + 1); the r
 __host__ __device__ inline int64_t chunk_offset(int64_t tmpl_off, int64_t w) {
     return 4 * tmpl_off + 1024 * w;
 }
-__host__ __device__ inline int32_t chunk_pad(int32_t q) {
+__host__ __device__ constexpr int32_t chunk_pad(int32_t q) {
     return (q + kKGroup - 1) / kKGroup * kKGroup;
 }
 
