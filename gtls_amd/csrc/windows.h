@@ -733,7 +733,7 @@ struct HPlanes {
 // FC: the chunk count F = d / q when known at compile time (8: every width with q >= 8 at
 // the default skip of 8), so that the diagonal reads take immediate offsets and the columns
 // past F are not read at all; 0 for any F. Direct: the sample operands' load layout (see
-// p_load; per-sample weights always take Quad).
+// p_load; Quad when false).
 template <bool ConstW, bool TileMean, bool BL, int FC = 0, bool Direct = ConstW>
 __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __restrict__ C,
                                              const char* __restrict__ C2, int32_t zrel,
@@ -796,14 +796,12 @@ __device__ __forceinline__ ArgMin mfma_width(const HPlanes& P, const char* __res
         const PSweep wa = sweep(P.X, C, st, TileMean, 0);
         if constexpr (ConstW) {
             super_tile_prows<BL, Direct>(wa, sweep(P.X, C, stn, TileMean, 0), ng, q, rows32, ops,
-                                       ring0,
-                                 ringA);
+                                         ring0, ringA);
         } else {
             const PSweep wb = sweep(P.W, C2, st, false, 1);
             super_tile_prows<BL, Direct>(wa, wb, ng, q, rows32, ops, ring0, ringA);
             super_tile_prows<BL, Direct>(wb, sweep(P.X, C, stn, false, 0), ng, q, rows32, ops,
-                                        ring0,
-                                 ringB);
+                                         ring0, ringB);
         }
         wave_sync_lds();
         // diagonal sums: column c at the lane's row rw + min(c, F) (rows rw .. rw + F, this
