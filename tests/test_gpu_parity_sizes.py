@@ -7,7 +7,8 @@ the C ABI (bar in tests/parity_util.py):
     cadences, two 30-min curves above 40,000 points), with a common dy and with per-sample
     dy, each curve on a 400-period sample of its own grid plus its injected period;
   * a curve with a sample time repeated 300 times, so that one phase bucket exceeds the
-    in-LDS window sort and the prep kernel takes its global sort form.
+    in-LDS window sort and the prep kernel takes its global sort form;
+  * a 70-day 20-s curve (N = 302,400) on periods whose widths exceed 32,768 samples.
 """
 import numpy as np
 import pytest
@@ -120,3 +121,42 @@ def test_repeated_times_take_the_global_sort(gpu, mode):
                                                                    skip_point=SKIP)]
     worst, n_same, n_ok = check(plan, mode, SKIP, per, klo, khi, c, k, s, "repeated")
     report("repeated-times-global-sort", mode, worst, n_same, n_ok)
+
+
+def fast_cadence_curve(seed=53):
+    """A 70-day curve at 20-s cadence (N = 302,400) with a 23-day transit. Its widest
+    durations span up to 36,288 samples (0.12 N), past 32,768."""
+    rng = np.random.default_rng(seed)
+    n = 302400
+    t = synthetic.T_START + np.arange(n) * (20.0 / 86400.0)
+    y = 1.0 + rng.normal(0.0, 1e-3, n)
+    y *= synthetic.transit_flux(t, 23.0, synthetic.T_START + 7.0, synthetic.solar_t14(23.0),
+                                2e-3)
+    return validate_inputs(t, y, None)
+
+
+def wide_width_sample(plan, k=12, seed=61):
+    """k random periods, those around the injected 23 d, and k spread over the periods whose
+    widest width exceeds 32,768 samples."""
+    wide = np.nonzero(plan.table.widths[plan.k_hi] > 32768)[0]
+    spread = wide[np.linspace(0, len(wide) - 1, k).astype(int)]
+    return np.union1d(period_sample(plan.periods, k, seed, around=23.0, n_around=2), spread)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32c"])
+def test_widths_past_32768_samples_match_oracle(gpu, mode):
+    """Widths wider than 32,768 samples: the zeroed template tail after the last template
+    row and the zero block of the chunked rows are sized from the widest width of the
+    search, and the sort takes about 120 windows per period (N = 302,400)."""
+    t, y, dy = fast_cadence_curve()
+    plan = S.make_plan(t, y, dy, validate_args({}))
+    assert plan.table.widths.max() > 36000
+    sel = wide_width_sample(plan)
+    per, klo, khi = plan.periods[sel], plan.k_lo[sel], plan.k_hi[sel]
+    assert np.sum(plan.table.widths[khi] > 32768) >= 12
+    c, k, s = [x.cpu().numpy() for x in S.DeviceCurve(plan).search(per, klo, khi, mode=mode,
+                                                                   skip_point=SKIP)]
+    worst, n_same, n_ok = check(plan, mode, SKIP, per, klo, khi, c, k, s, "wide widths")
+    report("wide-widths-20s", mode, worst, n_same, n_ok, "n=%d" % plan.n)
+    i = int(np.nanargmin(c))
+    assert per[i] == pytest.approx(23.0, rel=2e-3)
