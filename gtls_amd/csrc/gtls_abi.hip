@@ -3,6 +3,7 @@
 // allocates device memory or waits for the device.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -479,7 +480,8 @@ int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb, int32_t wg_
                     int32_t quad_q) {
     if (!ctx) return GTLS_E_SHAPE;
     if (toep_q < 0 || toep_q > kToepQ) return fail(ctx, GTLS_E_SHAPE, "toep_q out of [0, 24]");
-    if (chunk_mb < 0) return fail(ctx, GTLS_E_SHAPE, "chunk_mb < 0");
+    if (chunk_mb < 0 || chunk_mb > (INT64_MAX >> 20))
+        return fail(ctx, GTLS_E_SHAPE, "chunk_mb out of [0, 2^43)");
     if (wg_per_cu < 0) return fail(ctx, GTLS_E_SHAPE, "wg_per_cu < 0");
     if (quad_q < 0) return fail(ctx, GTLS_E_SHAPE, "quad_q < 0");
     ctx->wg_per_cu = wg_per_cu;

@@ -191,13 +191,14 @@ int gtls_set_graph_timing(gtls_ctx* ctx, uint64_t* stamps, int64_t max_chunks);
 /*
  * Tuning of the float32 search (no reference counterpart; set by gtls_amd.tune): strides q
  * below toep_q (0 .. 24, default 8) take the Toeplitz matrix-core tiles, larger ones the
- * diagonal-sum tiles; chunk_mb (0 keeps the current value) bounds the per-period buffers
- * of one period chunk; wg_per_cu (0: as many as fit) caps the resident workgroups per CU
+ * diagonal-sum tiles; chunk_mb (0 keeps the current value, below 2^43) bounds the
+ * per-period buffers of one period chunk; wg_per_cu (0: as many as fit) caps the resident workgroups per CU
  * of the persistent search kernels (their waves per SIMD); with a common weight, the
  * diagonal-sum tiles of strides q >= quad_q (0: the default, 96) load their sample
  * operands 8 rows x 128 bytes per load and move them to the matrix-core layout in
  * registers, shorter strides load them in that layout directly (the results are the same
- * either way). Applies to later calls on the context.
+ * either way; with per-sample weights the bound is fixed at 24). Applies to later calls on
+ * the context; an argument out of range returns GTLS_E_SHAPE and changes nothing.
  */
 int gtls_set_tuning(gtls_ctx* ctx, int32_t toep_q, int64_t chunk_mb, int32_t wg_per_cu,
                     int32_t quad_q);

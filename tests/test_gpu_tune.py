@@ -43,3 +43,19 @@ def test_autotune_picks_a_candidate_and_keeps_results(gpu):
             assert torch.equal(torch.nan_to_num(c, nan=-1.0), torch.nan_to_num(c1, nan=-1.0)), quad
     finally:
         tune.set_tuning(0, *tune.DEFAULT)
+
+
+def test_out_of_range_tuning_is_refused(gpu):
+    """gtls_set_tuning refuses a Toeplitz bound outside [0, 24], a negative or unshiftable
+    chunk budget, and negative workgroup counts or layout thresholds; the setting applied
+    before stays in force."""
+    from gtls_amd import _ffi
+    tune.set_tuning(0, *tune.DEFAULT)
+    try:
+        for bad in ((25, 0, 0, 0), (-1, 0, 0, 0), (8, -1, 0, 0), (8, 1 << 43, 0, 0),
+                    (8, 0, -1, 0), (8, 0, 0, -1)):
+            with pytest.raises(_ffi.GtlsError):
+                tune.set_tuning(0, *bad)
+            assert tune.applied(0) == tune.DEFAULT, bad
+    finally:
+        tune.set_tuning(0, *tune.DEFAULT)
