@@ -76,3 +76,31 @@ def test_device_spectra_match_host(gpu):
         assert abs(res["SDE"][i] - s_det) <= 1e-10 * abs(s_det)
         assert int(np.argmax(res["power"][i])) == int(np.argmax(np.asarray(p_det)))
     print("device spectra on %d of %d curves" % (res["device"].sum(), bp.n_curves))
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32c"])
+def test_batched_without_skip_matches_oracle(gpu, mode):
+    """The batched call with no T0 skip (T0_fit_margin 0, the form of the candidates'
+    re-search) on every 10th period of six curves, against the oracle at the shared bar
+    (tests/parity_util.py: index-exact in FP64, near ties only in FP32c)."""
+    from gtls_amd.grid import skip_factor
+    from gtls_amd.validate import validate_args, validate_inputs
+    from parity_util import check, report
+    none = skip_factor(0.0)
+    plans = []
+    for (t, y, dy, _) in synthetic.batch(6, seed=21, n_min=300, n_max=6000):
+        t, y, dy = validate_inputs(t, y, dy)
+        full = S.make_plan(t, y, dy, validate_args({}))
+        plans.append(S.make_plan(t, y, dy, validate_args({}), periods=full.periods[::10]))
+    bp = B.BatchPlan(plans)
+    db = B.DeviceBatch(bp)
+    db.search(mode, none)
+    chi2 = bp.unpack(db.chi2.cpu().numpy())
+    bk = bp.unpack(db.best_k.cpu().numpy())
+    bs = bp.unpack(db.best_s.cpu().numpy())
+    worst, n_same, n_ok = 0.0, 0, 0
+    for i, p in enumerate(plans):
+        w, ns, no = check(p, mode, none, p.periods, p.k_lo, p.k_hi, chi2[i], bk[i], bs[i],
+                          "curve %d (n=%d)" % (i, p.n))
+        worst, n_same, n_ok = max(worst, w), n_same + ns, n_ok + no
+    report("batched-no-skip", mode, worst, n_same, n_ok, "curves=%d" % len(plans))
