@@ -372,7 +372,6 @@ struct HSweep {
     const char* C;       // template pair plane (wave-uniform)
     uint32_t xo[NT];     // lane's A operand byte offsets, one per tile
     int32_t co;          // lane's B operand byte offset
-    lds_char* LB;        // lane's staged B operand of group 0 (BL)
 };
 
 // One K group's operands of a sweep: the sample pairs of the NT tiles and the template
@@ -388,14 +387,13 @@ struct HSet {
     h8 b3[Ones ? 1 : 0];
 };
 
-template <int NT, bool Ones, bool BL = false>
+template <int NT, bool Ones>
 __device__ __forceinline__ void h_load(const HSweep<NT>& w, int32_t g, HSet<NT, Ones>& o,
                                        int32_t d = 0) {
     const int32_t off = 4 * kKGroup * g;
 #pragma unroll
     for (int t = 0; t < NT; ++t) o.a[t] = ld_h8(w.X + (w.xo[t] + (uint32_t)off));
-    if constexpr (BL) o.b = *(const __attribute__((address_space(3))) h8*)(w.LB + 1024 * g);
-    else o.b = ld_h8(w.C + (int64_t)(w.co + off));
+    o.b = ld_h8(w.C + (int64_t)(w.co + off));
     if constexpr (Ones) {
         const int32_t tap = (w.co + off) / 4;   // exact: w.co and off are multiples of 4
         uint32_t v[4];
@@ -467,7 +465,7 @@ __device__ __forceinline__ void h_mma(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
 // does next. Two explicit operand sets with straight-line pairs of groups: each MFMA
 // block waits only for its own set while the other set's loads are in flight (the
 // scheduling barriers keep each load block ahead of the MFMA block before it).
-template <int NT, bool Ones, bool BL = false>
+template <int NT, bool Ones>
 __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT],
                                           const HSweep<NT>& cur, const HSweep<NT>& nxt,
                                           int32_t ng, HSet<NT, Ones>& s0,
@@ -480,7 +478,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
     }
     int32_t g = 0;
     for (; g + 1 < ng; g += 2) {
-        h_load<NT, Ones, BL>(cur, g + 1, s1, d);
+        h_load(cur, g + 1, s1, d);
         __builtin_amdgcn_sched_barrier(0);
         a_permute(s0, paddr);
         h_mma(acc, accw, s0);
@@ -494,8 +492,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
 #pragma unroll
         for (int t = 0; t < NT; ++t) w.xo[t] = more ? cur.xo[t] : nxt.xo[t];
         w.co = more ? cur.co : nxt.co;
-        if constexpr (BL) w.LB = more ? cur.LB : nxt.LB;
-        h_load<NT, Ones, BL>(w, more ? g + 2 : 0, s0, d);
+        h_load(w, more ? g + 2 : 0, s0, d);
         __builtin_amdgcn_sched_barrier(0);
         a_permute(s1, paddr);
         h_mma(acc, accw, s1);
@@ -505,7 +502,7 @@ __device__ __forceinline__ void run_sweep(mfma_v4 (&acc)[NT], mfma_v4 (&accw)[NT
         a_permute(s0, paddr);
         h_mma(acc, accw, s0);
         __builtin_amdgcn_sched_barrier(0);
-        h_load<NT, Ones, BL>(nxt, 0, s0, d);
+        h_load(nxt, 0, s0, d);
     }
 }
 
