@@ -99,16 +99,4 @@ __device__ __forceinline__ ArgMin wave_argmin(ArgMin a) {
     return a;
 }
 
-// Workgroup argmin; result valid in every thread. `tmp` holds blockDim.x / 64 entries.
-__device__ __forceinline__ ArgMin block_argmin(ArgMin a, ArgMin* tmp) {
-    a = wave_argmin(a);
-    if (lane_id() == 0) tmp[wave_id()] = a;
-    __syncthreads();
-    ArgMin r = tmp[0];
-    for (int w = 1; w < (int)(blockDim.x / kWave); ++w)
-        if (argmin_less(tmp[w], r)) r = tmp[w];
-    __syncthreads();
-    return r;
-}
-
 }  // namespace gtls
