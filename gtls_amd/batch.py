@@ -86,15 +86,6 @@ class BatchPlan:
             out[ci] = arr[self.per_off[pos]:self.per_off[pos + 1]]
         return out
 
-    def work_points(self):
-        """Sum over (curve, period) of N' = N + widest searched width (for roofline)."""
-        s = 0
-        for p in self.plans:
-            wmax = p.table.widths[np.clip(p.k_hi, 0, None)]
-            ok = p.k_lo <= p.k_hi
-            s += int(np.sum((p.n + wmax)[ok]))
-        return s
-
 
 def size_groups(bp, ratio=2.0):
     """Contiguous ranges [c0, c1) of packed curves (longest first) in which no curve is

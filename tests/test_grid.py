@@ -73,3 +73,23 @@ def test_skip_factor():
     assert grid.skip_factor(0.125) == 8
     assert grid.skip_factor(0.01) == 100
     assert grid.skip_factor(0) == C.SKIP_NONE
+
+
+def test_periodogram_is_the_grid_power_searches():
+    """gtls.periodogram (main.py:186-206) returns the period grid power() searches for the
+    same kwargs (in period_grid's own descending order; the search keeps it ascending), and
+    the user's periods unchanged when given."""
+    from gtls_amd.api import gtls
+    from gtls_amd.search import make_plan
+    from gtls_amd.validate import validate_args
+    rng = np.random.default_rng(3)
+    t = 1000.0 + np.arange(4000) / 144.0
+    y = 1.0 + rng.normal(0.0, 1e-3, t.size)
+    g = gtls(t, y, verbose=False)
+    kw = dict(oversampling_factor=2, period_min=0.7)
+    p = g.periodogram(**kw)
+    plan = make_plan(g.t, g.y, g.dy, validate_args(kw))
+    assert len(p) > 100 and np.all(np.diff(p) < 0)
+    assert np.array_equal(p[::-1], plan.periods)
+    user = np.array([1.1, 2.2, 3.3])
+    assert np.array_equal(g.periodogram(periods=user), user)
