@@ -96,3 +96,32 @@ def test_batched_without_skip_matches_oracle(gpu, mode):
                           "curve %d (n=%d)" % (i, p.n))
         worst, n_same, n_ok = max(worst, w), n_same + ns, n_ok + no
     report("batched-no-skip", mode, worst, n_same, n_ok, "curves=%d" % len(plans))
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32c"])
+def test_batch_mixing_common_and_per_sample_dy_matches_oracle(gpu, mode):
+    """One call over curves of both weight paths (every other curve with per-sample dy),
+    each on every 4th period, against the oracle at the shared bar."""
+    from gtls_amd.validate import validate_args, validate_inputs
+    from parity_util import check, report
+    rng = np.random.default_rng(77)
+    plans = []
+    for j, (t, y, dy, _) in enumerate(small_batch(8, seed=13)):
+        if j % 2:
+            dy = np.std(y) * rng.uniform(0.5, 1.5, len(t))
+        t, y, dy = validate_inputs(t, y, dy)
+        full = S.make_plan(t, y, dy, validate_args({}))
+        plans.append(S.make_plan(t, y, dy, validate_args({}), periods=full.periods[::4]))
+    assert len({bool(np.ptp(p.dy) > 0) for p in plans}) == 2
+    bp = B.BatchPlan(plans)
+    db = B.DeviceBatch(bp)
+    db.search(mode, 8)
+    chi2 = bp.unpack(db.chi2.cpu().numpy())
+    bk = bp.unpack(db.best_k.cpu().numpy())
+    bs = bp.unpack(db.best_s.cpu().numpy())
+    worst, n_same, n_ok = 0.0, 0, 0
+    for i, p in enumerate(plans):
+        w, ns, no = check(p, mode, 8, p.periods, p.k_lo, p.k_hi, chi2[i], bk[i], bs[i],
+                          "curve %d (n=%d)" % (i, p.n))
+        worst, n_same, n_ok = max(worst, w), n_same + ns, n_ok + no
+    report("batched-mixed-dy", mode, worst, n_same, n_ok, "curves=%d" % len(plans))
