@@ -16,6 +16,10 @@ Also measured in the same run (reported beside `value`, never as `value`):
   parity   -- on that sample: same best period (SDE peak) and same best width, SDE error
   cpu      -- the FP64 CPU oracle (OpenMP, all host cores) on a bounded sample of curves,
               extrapolated the same way, with the same peak/SDE check
+  single   -- with plain: the single-curve configurations (the 19,728-point 2-min curve,
+              the 4-year 29.4-min curve, the reference tutorial's 41,816-point shape) on
+              their whole default grids, one gtls_search call per mode; with cpu: the
+              oracle on the same grids (C2 on 1 % of its periods, extrapolated)
 
     python bench.py [--steps K] [--warmup W] [--curves 4096] [--batch 512]
 """
@@ -180,6 +184,122 @@ def refine_leg(args, plans, batch0, skip):
         "median chi2 ratio %.4f" % (mb.n_models, args.refine_steps, dt * 1e3 / args.refine_steps,
                                     out["ms_per_step"], out["chi2_ratio_median"]))
     return out
+
+
+# Single-curve workloads searched one curve per call (the plain path): key, description,
+# synthetic generator, share of the period grid the CPU oracle times (extrapolated by
+# window work when below 1), and published context.
+SINGLE_CURVES = (
+    ("C1", "19,728-point 2-min curve (27.4 d, one gap), whole default grid", "single_2min",
+     1.0, None),
+    ("C2", "4-year 29.4-min curve (17 gaps, red noise), whole default grid", "long_30min",
+     0.01, None),
+    ("tutorial_shape", "41,816 points at 58.85 s over 30.77 d, whole default grid (2,868 "
+     "periods of 0.601-15.386 d), the shape of the reference tutorial's Kepler search",
+     "kepler_sc_month", 1.0,
+     "the reference tutorial (tutorial/1-keplerTest.ipynb:102-162) reports about 1.7 s for "
+     "this search with GTLS 0.4.2 on an unstated GPU and about 7 s for TLS on 32 CPU threads"),
+)
+
+
+def single_curve_plans():
+    """Host plans (default grids) of the single-curve workloads, with their planning time."""
+    from gtls_amd import search as S
+    from gtls_amd import synthetic
+    from gtls_amd.validate import validate_args, validate_inputs
+    out = []
+    for key, what, fn, frac, context in SINGLE_CURVES:
+        t, y, dy, truth = getattr(synthetic, fn)()
+        t, y, dy = validate_inputs(t, y, dy)
+        ts = time.perf_counter()
+        plan = S.make_plan(t, y, dy, validate_args({}))
+        out.append(dict(key=key, what=what, plan=plan, truth=truth, frac=frac,
+                        context=context, plan_s=time.perf_counter() - ts))
+    return out
+
+
+def single_gpu_leg(singles, skip):
+    """Every single-curve workload's whole grid in one gtls_search call per mode (FP64,
+    FP32c), after one untimed call; wall-clock seconds of the call including the upload
+    of the per-period arrays. Returns (JSON figures, device results per key)."""
+    import torch
+    from gtls_amd import search as S
+    res, arrays = {}, {}
+    for sc in singles:
+        p = sc["plan"]
+        dc = S.DeviceCurve(p, 0)
+        dc.search(p.periods, p.k_lo, p.k_hi, mode="fp32c", skip_point=skip)
+        torch.cuda.synchronize()
+        r = {"what": sc["what"], "points": int(p.n), "periods": int(len(p.periods)),
+             "widths": int(len(p.table.widths)),
+             "period_range_d": [round(float(p.periods.min()), 4),
+                                round(float(p.periods.max()), 4)],
+             "host_planning_s": round(sc["plan_s"], 3)}
+        got = {}
+        for mode in ("fp64", "fp32c"):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            c, k, s = dc.search(p.periods, p.k_lo, p.k_hi, mode=mode, skip_point=skip)
+            torch.cuda.synchronize()
+            r[mode + "_search_s"] = round(time.perf_counter() - ts, 4)
+            got[mode] = tuple(x.cpu().numpy() for x in (c, k, s))
+        c64, k64, s64 = got["fp64"]
+        c32, k32, s32 = got["fp32c"]
+        ok = np.isfinite(c64)
+        r["fp32c_max_rel_chi2_vs_fp64"] = float(np.max(np.abs(c32[ok] - c64[ok]) / c64[ok]))
+        r["fp32c_same_index_as_fp64"] = "%d/%d" % (
+            int(np.sum(((k32 == k64) & (s32 == s64))[ok])), int(ok.sum()))
+        r["best_period_d"] = {m: float(p.periods[int(np.nanargmin(got[m][0]))])
+                              for m in ("fp64", "fp32c")}
+        if sc["truth"] is not None:
+            r["injected_period_d"] = float(sc["truth"]["period"])
+        if sc["context"]:
+            r["context"] = sc["context"]
+        res[sc["key"]] = r
+        arrays[sc["key"]] = got["fp64"]
+        log("single %s: %d points, %d periods: fp64 %.3f s, fp32c %.3f s, best %.4f d" % (
+            sc["key"], p.n, len(p.periods), r["fp64_search_s"], r["fp32c_search_s"],
+            r["best_period_d"]["fp32c"]))
+        del dc
+        torch.cuda.synchronize()
+    return res, arrays
+
+
+def single_cpu_leg(singles, skip, cores, gpu_res, gpu_arrays):
+    """The FP64 CPU oracle on every single-curve workload (its whole grid, or an evenly
+    spaced share of its periods extrapolated by window work), with the device FP64 result
+    checked on the same periods."""
+    import dataclasses
+    from oracle import oracle
+    res = {}
+    for sc in singles:
+        p = sc["plan"]
+        n_per = len(p.periods)
+        m = max(1, int(round(sc["frac"] * n_per)))
+        sel = np.unique(np.linspace(0, n_per - 1, m).astype(np.int64))
+        per, klo, khi = p.periods[sel], p.k_lo[sel], p.k_hi[sel]
+        ts = time.perf_counter()
+        c, k, s = oracle.search_plan(p, skip_point=skip, periods=per, k_lo=klo, k_hi=khi,
+                                     nthreads=cores)
+        dt = time.perf_counter() - ts
+        sub = dataclasses.replace(p, periods=per, k_lo=klo, k_hi=khi)
+        full_s = dt * window_macs(p, skip) / max(window_macs(sub, skip), 1.0)
+        r = {"sample_periods": int(len(sel)), "sample_s": round(dt, 3),
+             "full_grid_s": round(full_s, 3), "extrapolated": bool(len(sel) < n_per),
+             "threads": cores}
+        if sc["key"] in gpu_arrays:
+            c64, k64, s64 = (x[sel] for x in gpu_arrays[sc["key"]])
+            ok = np.isfinite(c)
+            r["gpu_fp64_same_index"] = "%d/%d" % (
+                int(np.sum(((k64 == k) & (s64 == s))[ok])), int(ok.sum()))
+            r["gpu_fp64_max_rel_chi2"] = float(np.max(np.abs(c64[ok] - c[ok]) / c[ok]))
+            g = gpu_res[sc["key"]]
+            r["gpu_fp64_speedup"] = round(full_s / g["fp64_search_s"], 2)
+            r["gpu_fp32c_speedup"] = round(full_s / g["fp32c_search_s"], 2)
+        res[sc["key"]] = r
+        log("cpu single %s: %d of %d periods in %.2f s -> %.2f s whole grid" % (
+            sc["key"], len(sel), n_per, dt, full_s))
+    return res
 
 
 def stratified(work, k):
@@ -526,6 +646,7 @@ def main():
         log("weighted: %d curves in %.2f s, best period %d/%d vs fp64, max rel chi2 %.2e" % (
             k, tw, sp, k, mrel))
 
+    singles, single_res, single_arrays = [], {}, {}
     if not args.no_plain:
         sample = [int(done[i]) for i in stratified(macs[done], args.plain_sample)]
         t_plain = w_plain = 0.0
@@ -555,6 +676,10 @@ def main():
                       "parity_vs_plain_max_rel_sde_err": float(max_dsde)})
         log("plain fp64: %.1f curves/h (sample %d), parity %d/%d %d/%d, max dSDE %.2e" % (
             plain_value, len(sample), same_p, len(sample), same_w, len(sample), max_dsde))
+        # the single-curve configurations, one curve per call
+        singles = single_curve_plans()
+        single_res, single_arrays = single_gpu_leg(singles, skip)
+        extra["single_curve"] = single_res
 
     if not args.no_refine:
         extra["refine"] = refine_leg(args, plans, batches[0], skip)
@@ -593,6 +718,9 @@ def main():
                          "extrapolated by window MACs" % (
                              len(used), min(plans[i].n for i in used),
                              max(plans[i].n for i in used), t_cpu, cores)}
+        if singles:
+            cpu["single_curve"] = single_cpu_leg(singles, skip, cores, single_res,
+                                                 single_arrays)
         extra.update({"parity_vs_cpu_same_best_period": "%d/%d" % (same_p, len(used)),
                       "parity_vs_cpu_same_best_width": "%d/%d" % (same_w, len(used)),
                       "parity_vs_cpu_max_rel_sde_err": float(max_dsde)})

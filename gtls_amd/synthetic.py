@@ -9,6 +9,8 @@ t = 1000 d so that the fold sees realistic magnitudes of t / P.
   long_30min():   4 years at 29.4-min cadence with 17 gaps of 1-5 d (~65,000 points), AR(1)
                   red noise (phi 0.7, sigma 3e-4) plus white 2e-4, P = 45.3 d, depth 5e-4,
                   T14 = 6.1 h
+  kepler_sc_month(): one month at 58.85-s cadence (41,816 points, 30.772 d, one gap):
+                  the shape of the reference tutorial's Kepler search (2,868 periods)
   batch():        n curves, N log-uniform in [2000, 70000], cadence 2 or 30 min, noise
                   U(1e-4, 2e-3), half with a transit (P U(0.8, 60) d, depth U(2e-4, 5e-3),
                   T14 of a central transit across a solar-density star)
@@ -74,6 +76,24 @@ def long_30min(seed=3):
     t0 = T_START + rng.uniform(0, P)
     y *= transit_flux(t, P, t0, t14, depth)
     return t, y, None, dict(period=P, t0=t0, t14=t14, depth=depth)
+
+
+def kepler_sc_month(seed=5):
+    """The shape of the reference tutorial's Kepler curve (tutorial/1-keplerTest.ipynb:
+    41,816 points searched over 2,868 periods of 0.601-15.386 d with the default grid): one
+    month at Kepler's 58.85-s short cadence (30.772 d) with one 2.3-d gap, white noise
+    3e-4, a transit at P = 2.47 d of depth 1e-3."""
+    rng = np.random.default_rng(seed)
+    cad = 58.85 / 86400.0
+    n_full = int(round(30.772 / cad)) + 1
+    t = T_START + np.arange(n_full) * cad
+    keep = np.ones(n_full, dtype=bool)
+    keep[20000:20000 + n_full - 41816] = False
+    t = t[keep]
+    P, depth = 2.47, 1e-3
+    t0 = T_START + rng.uniform(0, P)
+    y = (1.0 + rng.normal(0.0, 3e-4, t.size)) * transit_flux(t, P, t0, solar_t14(P), depth)
+    return t, y, None, dict(period=P, t0=t0, t14=solar_t14(P), depth=depth)
 
 
 def solar_t14(period):

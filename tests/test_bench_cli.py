@@ -29,3 +29,21 @@ def test_world_size_above_one_is_refused():
 def test_curves_must_fill_whole_batches():
     r = _run("--curves", "100", "--batch", "64")
     assert r.returncode == 2 and "multiple of --batch" in r.stderr
+
+
+def test_single_curve_workloads_have_their_stated_shapes():
+    """The single-curve legs search the stated workloads: the 19,728-point 2-min curve, the
+    4-year 29.4-min curve (~1.9e5 periods up to 730 d) and the reference tutorial's shape
+    (41,816 points, 2,868 periods of 0.601-15.386 d on the default grid)."""
+    import warnings
+    sys.path.insert(0, ROOT)
+    import bench
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        got = {sc["key"]: sc["plan"] for sc in bench.single_curve_plans()}
+    assert got["C1"].n == 19728
+    assert got["C2"].n > 65000 and len(got["C2"].periods) > 150000
+    tut = got["tutorial_shape"]
+    assert tut.n == 41816 and len(tut.periods) == 2868
+    assert round(float(tut.periods.min()), 3) == 0.601
+    assert round(float(tut.periods.max()), 3) == 15.386
