@@ -243,6 +243,9 @@ def single_gpu_leg(singles, skip):
             torch.cuda.synchronize()
             r[mode + "_search_s"] = round(time.perf_counter() - ts, 4)
             got[mode] = tuple(x.cpu().numpy() for x in (c, k, s))
+        # useful window FLOPs and algorithmic bytes of the call, as for the batched set
+        r["fp32c_window_tflops"] = round(2 * window_macs(p, skip) / r["fp32c_search_s"] / 1e12, 2)
+        r["fp32c_algorithmic_GBps"] = round(algorithmic_bytes(p) / r["fp32c_search_s"] / 1e9, 1)
         c64, k64, s64 = got["fp64"]
         c32, k32, s32 = got["fp32c"]
         ok = np.isfinite(c64)
