@@ -562,6 +562,7 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
         }
     }
     for (int w = tid; w < K; w += nt) wcur[w] = ws[w];
+    if (tid == 0) counts[1] = 0;   // the first window's 17+-key bucket count
     __syncthreads();
     // one pass: every sample's index into its window's range of `binned`, its bucket folded
     // again (the times are shared by the curve's periods and stay in cache; a stored bucket
@@ -610,8 +611,6 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
     for (int w = 0; w < K; ++w) {
         const int bfirst = wb[w], bnext = wb[w + 1];
         const int wstart = ws[w], wcnt = ws[w + 1] - wstart;
-        if (tid == 0) counts[1] = 0;
-        __syncthreads();
         // stage the window's samples at their bucket positions (indices, then times, of
         // all of a thread's samples in flight before the first use)
         {
@@ -666,6 +665,9 @@ __device__ __forceinline__ bool wg_fold_sort_windows(
             }
         }
         __syncthreads();
+        // counts[1] is not read again in this window; the barrier after the sink orders the
+        // reset before the next window's 17+-key buckets are counted
+        if (tid == 0) counts[1] = 0;
         sink(wstart, wcnt);
         __syncthreads();
     }
